@@ -1,0 +1,967 @@
+// C-ABI of the constraint-path walker path: context, walker sets and the
+// batched drop-ins for the reference's per-walker calls (see
+// include/afqmc_mi355x.h for the reference function each entry replaces).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/afqmc_mi355x.h"
+#include "cp_kernels.hpp"
+#include "gemm.hpp"
+#include "runtime.hpp"
+#include "small_dense.hpp"
+
+using cd = std::complex<double>;
+
+// ------------------------------------------------------------------ NiupNidn constants (host)
+// solveCoshxEqExpy / solveCosxEqExpy / exp2by2Matrix, simple_fun.cpp:7-91
+// Written out in real arithmetic on the branches the reference's libm (glibc) takes for
+// an argument with +0 imaginary part: sqrt(-a + 0i) = +i sqrt(a) and
+// acos(x + 0i) = -i acosh(x) for x > 1.  (The complex overloads of the host compiler
+// used with hipcc return +i acosh(x) there, which flips gamma of hopSpin.)
+static cd solve_cosh_exp(double y) {
+  const double e = std::exp(y);
+  cd g;
+  if (e * e - 1.0 >= 0.0) {
+    g = cd(std::log(e - std::sqrt(e * e - 1.0)), 0.0);
+  } else {
+    const double s = std::sqrt(1.0 - e * e);  // sqrt(e^2 - 1) = i s
+    g = cd(std::log(std::hypot(e, s)), std::atan2(-s, e));
+  }
+  return std::abs(g.real()) < std::abs(g.imag()) ? cd(0, g.imag()) : cd(g.real(), 0);
+}
+static cd solve_cos_exp(double y) {
+  const double e = std::exp(y);
+  cd g = (e <= 1.0) ? cd(std::acos(e), 0.0) : cd(0.0, -std::acosh(e));
+  return std::abs(g.real()) < std::abs(g.imag()) ? cd(0, g.imag()) : cd(g.real(), 0);
+}
+static void exp2by2(double& a, double& b, cd& c) {
+  if (std::abs(c) < 1e-60) { a = std::exp(a); b = std::exp(b); c = 0.0; return; }
+  double ac = std::abs(c), arg = std::arg(c);
+  double d0, d1;
+  cd v00, v01, v10, v11, im(0, 1);
+  if (std::abs(a - b) < 1e-60) {
+    double n = 1.0 / std::sqrt(2.0);
+    d0 = std::exp(-ac + a); d1 = std::exp(ac + b);
+    v00 = -n; v10 = n * std::exp(im * arg); v01 = n; v11 = n * std::exp(im * arg);
+  } else {
+    double xi = 0.5 * std::atan(2.0 * ac / (a - b));
+    d0 = std::exp(0.5 * std::cos(2 * xi) * (a - b) + std::sin(2 * xi) * ac + (a + b) * 0.5);
+    d1 = std::exp(-0.5 * std::cos(2 * xi) * (a - b) - std::sin(2 * xi) * ac + (a + b) * 0.5);
+    v00 = std::exp(-im * arg * 0.5) * std::cos(xi);
+    v10 = std::exp(im * arg * 0.5) * std::sin(xi);
+    v01 = -std::exp(-im * arg * 0.5) * std::sin(xi);
+    v11 = std::exp(im * arg * 0.5) * std::cos(xi);
+  }
+  a = (d0 * v00 * std::conj(v00) + d1 * v01 * std::conj(v01)).real();
+  b = (d0 * v10 * std::conj(v10) + d1 * v11 * std::conj(v11)).real();
+  c = d0 * v10 * std::conj(v00) + d1 * v11 * std::conj(v01);
+}
+
+struct afqmc_cp_s {
+  afqmc_ctx ctx = nullptr;
+  int device = 0;  // destroy must not touch ctx, which may already be gone
+  int L = 0, N = 0, Nup = 0, Ndn = 0, nblk = 1, dense = 1, fp32 = 0;
+  int Mb[2] = {0, 0}, Nb[2] = {0, 0}, row0[2] = {0, 0}, col0[2] = {0, 0};
+  int decomp = 1, force_type = 0, diag_only = 1, maxw = 0;
+  double dt = 0.01, cap = 1.5, log_half_L = 0.0, dtUSum = 0.0;
+  zd trial_logw{0, 0};
+  zd hop_logw[3];
+  DevBuf<char> phiT[2], expK[3][2], wdkT[2];
+  DevBuf<zd> gamma, cdiag;
+  DevBuf<double> U, mu, hx, hy, hz, cforce;
+  DevBuf<DevScalars> sc;
+  DevBuf<double> ubuf;
+  // workspace
+  DevBuf<char> ovl[2], scale[2], d2x2, qrS[2], qr64[2], qr64b[2];
+  DevBuf<zd> ldet[2], qrld[2], logO, gpart[3][2], per_walker, site_acc, den;
+  DevBuf<int> info[2], qrinfo[2], prefix, table, comb_num;
+  DevBuf<double> weight, comb_prob, force_rec;
+  DevBuf<int> aux_rec;
+  int ntile[2] = {1, 1};
+  std::vector<cd> h_gamma, h_cdiag;
+  size_t esz() const { return fp32 ? sizeof(zf) : sizeof(zd); }
+};
+
+struct afqmc_walkers_s {
+  afqmc_cp cp = nullptr;
+  int device = 0;  // destroy must not touch cp, which may already be gone
+  int nw = 0;
+  DevBuf<char> wf[2], tmp[2];
+  DevBuf<zd> logw, logw_tmp;
+  DevBuf<int> alive;
+  long long wsize(int b) const { return (long long)cp->Mb[b] * cp->Nb[b]; }
+};
+
+// ------------------------------------------------------------------ helpers
+namespace {
+
+template <typename T>
+void pack_block(const double* dense, int L2, int N, int r0, int c0, int Mb, int Nb, std::vector<cplx<T>>& out,
+                long long off) {
+  for (int j = 0; j < Nb; ++j)
+    for (int i = 0; i < Mb; ++i) {
+      const double* s = dense + 2 * ((long long)(r0 + i) + (long long)(c0 + j) * L2);
+      out[off + i + (long long)j * Mb] = cmk<T>((T)s[0], (T)s[1]);
+    }
+}
+
+template <typename T>
+int upload_matrix_blocks(afqmc_cp cp, const double* dense, int cols, DevBuf<char>* dst, bool square) {
+  const int L2 = 2 * cp->L;
+  for (int b = 0; b < cp->nblk; ++b) {
+    const int Mb = cp->Mb[b], Nb = square ? cp->Mb[b] : cp->Nb[b];
+    const int c0 = square ? cp->row0[b] : cp->col0[b];
+    std::vector<cplx<T>> h((size_t)Mb * Nb);
+    pack_block<T>(dense, L2, cols, cp->row0[b], c0, Mb, Nb, h, 0);
+    int rc = dst[b].alloc_bytes(h.size() * sizeof(cplx<T>));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(dst[b].p, h.data(), h.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+template <typename T>
+GemmArgs gemm_base() {
+  GemmArgs a;
+  memset(&a, 0, sizeof(a));
+  a.alpha_re = 1.0;
+  a.scale_group = 1;
+  return a;
+}
+
+template <typename T>
+int cp_gemm(afqmc_cp cp, int opa, int epi, const GemmArgs& a) {
+  return launch_cgemm<T>(opa, epi, a, cp->ctx->stream);
+}
+
+// overlap O_b = phiT_b^H W_b for all walkers ("giant" GEMM with shared A)
+template <typename T>
+int overlap_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf) {
+  for (int b = 0; b < cp->nblk; ++b) {
+    GemmArgs a = gemm_base<T>();
+    a.M = cp->Nb[b]; a.N = cp->Nb[b] * w->nw; a.K = cp->Mb[b]; a.batch = 1;
+    a.A = cp->phiT[b].p; a.lda = cp->Mb[b]; a.strideA = 0;
+    a.B = wf[b].p; a.ldb = cp->Mb[b]; a.strideB = 0;
+    a.C = cp->ovl[b].p; a.ldc = cp->Nb[b]; a.strideC = 0;
+    int rc = cp_gemm<T>(cp, OP_C, EPI_STORE, a);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+template <typename T>
+int gj_blocks(afqmc_cp cp, afqmc_walkers w, bool write_inverse, const int* alive) {
+  for (int b = 0; b < cp->nblk; ++b) {
+    SmallDenseArgs s;
+    memset(&s, 0, sizeof(s));
+    s.N = cp->Nb[b]; s.batch = w->nw;
+    s.in = cp->ovl[b].p; s.ld_in = cp->Nb[b]; s.stride_in = (long long)cp->Nb[b] * cp->Nb[b];
+    s.out = cp->ovl[b].p; s.ld_out = s.ld_in; s.stride_out = s.stride_in;
+    s.logdet = cp->ldet[b].p; s.info = cp->info[b].p; s.write_inverse = write_inverse ? 1 : 0;
+    s.alive = alive; s.alive_div = 1;
+    int rc = launch_gj_inverse<T>(s, cp->ctx->stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// X_b = W_b O_b^{-1} with green row-dot epilogue
+template <typename T>
+int green_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, const int* alive, bool kinetic) {
+  for (int b = 0; b < cp->nblk; ++b) {
+    GemmArgs a = gemm_base<T>();
+    a.M = cp->Mb[b]; a.N = cp->Nb[b]; a.K = cp->Nb[b]; a.batch = w->nw;
+    a.A = wf[b].p; a.lda = cp->Mb[b]; a.strideA = w->wsize(b);
+    a.B = cp->ovl[b].p; a.ldb = cp->Nb[b]; a.strideB = (long long)cp->Nb[b] * cp->Nb[b];
+    a.P0 = cp->phiT[b].p; a.P2 = kinetic ? cp->wdkT[b].p : nullptr; a.ldp = cp->Mb[b];
+    a.partner_half = cp->dense ? cp->L : 0;
+    a.out0 = cp->gpart[0][b].p; a.out1 = cp->dense ? cp->gpart[1][b].p : nullptr;
+    a.out2 = kinetic ? cp->gpart[2][b].p : nullptr;
+    a.alive = alive;
+    int rc = cp_gemm<T>(cp, OP_N, EPI_GREEN, a);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// W'_b = B_b diag(s) W_b  (all walkers, one GEMM per block)
+template <typename T>
+int propagate_gemm(afqmc_cp cp, afqmc_walkers w, int which, DevBuf<char>* in, DevBuf<char>* out, bool scaled) {
+  for (int b = 0; b < cp->nblk; ++b) {
+    GemmArgs a = gemm_base<T>();
+    a.M = cp->Mb[b]; a.N = cp->Nb[b] * w->nw; a.K = cp->Mb[b]; a.batch = 1;
+    a.A = cp->expK[which][b].p; a.lda = cp->Mb[b];
+    a.B = in[b].p; a.ldb = cp->Mb[b];
+    a.C = out[b].p; a.ldc = cp->Mb[b];
+    if (scaled) {
+      a.scale = cp->scale[b].p; a.scale_group = cp->Nb[b]; a.scale_strideG = cp->Mb[b]; a.scale_strideB = 0;
+    }
+    int rc = cp_gemm<T>(cp, OP_N, EPI_STORE, a);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+template <typename T>
+LogOArgs logo_args(afqmc_cp cp, afqmc_walkers w, int mode) {
+  LogOArgs l;
+  memset(&l, 0, sizeof(l));
+  l.nw = w->nw; l.nblk = cp->nblk; l.combine = COMBINE_WRAP; l.mode = mode;
+  for (int b = 0; b < cp->nblk; ++b) { l.ldet[b] = cp->ldet[b].p; l.info[b] = cp->info[b].p; }
+  l.logw = w->logw.p; l.logw_out = w->logw.p; l.alive = w->alive.p; l.trial_logw = cp->trial_logw;
+  l.logO = cp->logO.p; l.weight = cp->weight.p; l.den = cp->den.p;
+  return l;
+}
+
+template <typename T>
+int do_overlap(afqmc_cp cp, afqmc_walkers w, bool inverse) {
+  int rc = overlap_gemm<T>(cp, w, w->wf);
+  if (!rc) rc = gj_blocks<T>(cp, w, inverse, w->alive.p);
+  return rc;
+}
+
+template <typename T>
+HsSampleArgs hs_args(afqmc_cp cp, afqmc_walkers w) {
+  HsSampleArgs h;
+  memset(&h, 0, sizeof(h));
+  h.nw = w->nw; h.L = cp->L; h.decomp = cp->decomp; h.force_type = cp->force_type; h.nblk = cp->nblk;
+  h.dense = cp->dense;
+  for (int b = 0; b < cp->nblk; ++b) {
+    h.gd[b] = cp->gpart[0][b].p; h.go[b] = cp->dense ? cp->gpart[1][b].p : nullptr;
+    h.ntile[b] = cp->ntile[b]; h.Mb[b] = cp->Mb[b];
+    h.scale[b] = cp->scale[b].p;
+  }
+  h.gamma = cp->gamma.p; h.cdiag = cp->cdiag.p; h.const_force = cp->cforce.p;
+  h.cap = cp->cap; h.dt = cp->dt; h.log_half_L = cp->log_half_L; h.dtUSum = cp->dtUSum;
+  h.onebody_logw = cp->hop_logw[0];
+  h.alive = w->alive.p; h.prefix = cp->prefix.p; h.sc = cp->sc.p;
+  h.ubuf = cp->ubuf.p; h.logw = w->logw.p; h.diag_only = cp->diag_only; h.d2x2 = cp->d2x2.p;
+  h.fp32 = cp->fp32;
+  return h;
+}
+
+template <typename T>
+int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out) {
+  hipStream_t s = cp->ctx->stream;
+  int rc = do_overlap<T>(cp, w, true);
+  if (rc) return rc;
+  LogOArgs l = logo_args<T>(cp, w, LOGO_STEP);
+  if ((rc = launch_logo(l, s))) return rc;
+  if ((rc = launch_alive_scan(w->nw, w->alive.p, cp->prefix.p, cp->sc.p, s))) return rc;
+  if ((rc = green_gemm<T>(cp, w, w->wf, w->alive.p, false))) return rc;
+  HsSampleArgs h = hs_args<T>(cp, w);
+  if (force_out || aux_out) {
+    h.force_out = cp->force_rec.p;
+    h.aux_out = cp->aux_rec.p;
+  }
+  if ((rc = launch_hs_sample(h, s))) return rc;
+  if ((rc = launch_advance_counter(cp->sc.p, cp->L, s))) return rc;
+  if (cp->diag_only) {
+    if ((rc = propagate_gemm<T>(cp, w, 0, w->wf, w->tmp, true))) return rc;
+  } else {
+    if ((rc = launch_hs_apply_2x2<T>(w->nw, cp->L, cp->N, w->wf[0].p, w->tmp[0].p, cp->d2x2.p, s))) return rc;
+    if ((rc = propagate_gemm<T>(cp, w, 0, w->tmp, w->wf, false))) return rc;
+    // result already in wf
+    if (force_out || aux_out) goto records;
+    return 0;
+  }
+  for (int b = 0; b < cp->nblk; ++b) std::swap(w->wf[b], w->tmp[b]);
+records:
+  if (force_out) HIP_TRY(hipMemcpyAsync(force_out, cp->force_rec.p, sizeof(double) * w->nw * cp->L, hipMemcpyDeviceToHost, s));
+  if (aux_out) HIP_TRY(hipMemcpyAsync(aux_out, cp->aux_rec.p, sizeof(int) * w->nw * cp->L, hipMemcpyDeviceToHost, s));
+  if (force_out || aux_out) HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+// CholeskyQR2 in FP64 of each block: W <- W R^{-1}, logw += sum log R_kk (both passes)
+int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tmp) {
+  hipStream_t s = cp->ctx->stream;
+  int rc;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int b = 0; b < cp->nblk; ++b) {
+      const int Mb = cp->Mb[b], Nb = cp->Nb[b];
+      GemmArgs a = gemm_base<double>();
+      a.M = Nb; a.N = Nb; a.K = Mb; a.batch = w->nw;
+      a.A = wf[b].p; a.lda = Mb; a.strideA = w->wsize(b);
+      a.B = wf[b].p; a.ldb = Mb; a.strideB = w->wsize(b);
+      a.C = cp->qrS[b].p; a.ldc = Nb; a.strideC = (long long)Nb * Nb;
+      a.alive = w->alive.p;
+      if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
+      SmallDenseArgs c;
+      memset(&c, 0, sizeof(c));
+      c.N = Nb; c.batch = w->nw; c.in = cp->qrS[b].p; c.ld_in = Nb; c.stride_in = (long long)Nb * Nb;
+      c.out = cp->qrS[b].p; c.ld_out = Nb; c.stride_out = c.stride_in;
+      c.logdet = cp->qrld[b].p; c.info = cp->qrinfo[b].p; c.nsub = Nb; c.alive = w->alive.p; c.alive_div = 1;
+      if ((rc = launch_chol_inverse<double>(c, s))) return rc;
+      GemmArgs g = gemm_base<double>();
+      g.M = Mb; g.N = Nb; g.K = Nb; g.batch = w->nw;
+      g.A = wf[b].p; g.lda = Mb; g.strideA = w->wsize(b);
+      g.B = cp->qrS[b].p; g.ldb = Nb; g.strideB = (long long)Nb * Nb;
+      g.C = tmp[b].p; g.ldc = Mb; g.strideC = w->wsize(b);
+      g.alive = w->alive.p;
+      if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
+      std::swap(wf[b], tmp[b]);
+      if ((rc = launch_logw_add_real(w->nw, w->logw.p, w->alive.p, cp->qrld[b].p, nullptr, 0, 1.0, 0.0, s))) return rc;
+    }
+  }
+  return 0;
+}
+
+int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
+  if (!cp->fp32) return qr_impl_f64(cp, w, w->wf, w->tmp);
+  // mixed path: QR in FP64 on a widened copy of the FP32 walkers
+  hipStream_t s = cp->ctx->stream;
+  int rc;
+  for (int b = 0; b < cp->nblk; ++b) {
+    if ((rc = launch_convert_f32_f64(w->wf[b].p, cp->qr64[b].p, w->wsize(b) * w->nw, s))) return rc;
+  }
+  if ((rc = qr_impl_f64(cp, w, cp->qr64, cp->qr64b))) return rc;
+  for (int b = 0; b < cp->nblk; ++b) {
+    if ((rc = launch_convert_f64_f32(cp->qr64[b].p, w->wf[b].p, w->wsize(b) * w->nw, s))) return rc;
+  }
+  return 0;
+}
+
+template <typename T>
+int pop_control_impl(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats) {
+  hipStream_t s = cp->ctx->stream;
+  int rc = do_overlap<T>(cp, w, false);
+  if (rc) return rc;
+  LogOArgs l = logo_args<T>(cp, w, LOGO_WEIGHT);
+  if ((rc = launch_logo(l, s))) return rc;
+  if ((rc = launch_comb(w->nw, cp->weight.p, cp->ubuf.p, cp->sc.p, cp->table.p, cp->comb_prob.p, cp->comb_num.p, s)))
+    return rc;
+  void* wfp[2] = {w->wf[0].p, cp->nblk > 1 ? w->wf[1].p : nullptr};
+  long long ws[2] = {w->wsize(0), cp->nblk > 1 ? w->wsize(1) : 0};
+  if ((rc = launch_apply_table<T>(w->nw, cp->nblk, wfp, ws, w->logw.p, w->alive.p, cp->table.p, s))) return rc;
+  if (weights_out) HIP_TRY(hipMemcpyAsync(weights_out, cp->weight.p, sizeof(double) * w->nw, hipMemcpyDeviceToHost, s));
+  if (table_out) HIP_TRY(hipMemcpyAsync(table_out, cp->table.p, sizeof(int) * w->nw, hipMemcpyDeviceToHost, s));
+  if (stats) {
+    DevScalars h;
+    HIP_TRY(hipMemcpyAsync(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    stats[0] = h.wmin; stats[1] = h.wmax; stats[2] = h.wavg;
+  }
+  if (weights_out || table_out) HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+template <typename T>
+int measure_impl(afqmc_cp cp, afqmc_walkers w) {
+  hipStream_t s = cp->ctx->stream;
+  int rc;
+  if ((rc = propagate_gemm<T>(cp, w, 2, w->wf, w->tmp, false))) return rc;
+  if ((rc = launch_logw_shift(w->nw, w->logw.p, w->logw_tmp.p, cp->hop_logw[2], s))) return rc;
+  if ((rc = overlap_gemm<T>(cp, w, w->tmp))) return rc;
+  if ((rc = gj_blocks<T>(cp, w, true, w->alive.p))) return rc;
+  LogOArgs l = logo_args<T>(cp, w, LOGO_MEASURE);
+  l.logw = w->logw_tmp.p;
+  if ((rc = launch_logo(l, s))) return rc;
+  if ((rc = green_gemm<T>(cp, w, w->tmp, w->alive.p, true))) return rc;
+  EnergyArgs e;
+  memset(&e, 0, sizeof(e));
+  e.nw = w->nw; e.L = cp->L; e.nblk = cp->nblk; e.dense = cp->dense;
+  for (int b = 0; b < cp->nblk; ++b) {
+    e.gd[b] = cp->gpart[0][b].p; e.go[b] = cp->dense ? cp->gpart[1][b].p : nullptr; e.gk[b] = cp->gpart[2][b].p;
+    e.ntile[b] = cp->ntile[b]; e.Mb[b] = cp->Mb[b];
+  }
+  e.U = cp->U.p; e.mu = cp->mu.p; e.hx = cp->hx.p; e.hy = cp->hy.p; e.hz = cp->hz.p;
+  e.alive = w->alive.p; e.den = cp->den.p; e.per_walker = cp->per_walker.p; e.site_acc = cp->site_acc.p; e.sc = cp->sc.p;
+  return launch_energy(e, s);
+}
+
+}  // namespace
+
+#define DISPATCH(cp, fn, ...) ((cp)->fp32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+
+// ------------------------------------------------------------------ C entry points
+extern "C" {
+
+int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
+  REQUIRE(ctx && d && out, "afqmc_cp_create: null argument");
+  REQUIRE(d->L > 0 && d->N > 0 && d->max_walkers > 0, "afqmc_cp_create: L, N, max_walkers must be positive");
+  REQUIRE(d->K && d->exp_minus_dt_k && d->exp_minus_half_dt_k && d->exp_half_dt_k && d->phiT && d->U,
+          "afqmc_cp_create: K, propagators, phiT and U are required");
+  REQUIRE(d->decomp_type >= 0 && d->decomp_type <= 3, "Can not find the matched decompType! %d", d->decomp_type);
+  REQUIRE(d->force_type == 0 || d->force_type == 1, "Do not know method.forceType %d", d->force_type);
+  REQUIRE(d->force_cap >= 0.0, "forceCap must be positive or zero!");
+  ScopedDevice sd(ctx);
+  afqmc_cp cp = new afqmc_cp_s();
+  cp->ctx = ctx;
+  cp->device = ctx->device;
+  cp->L = d->L; cp->N = d->N; cp->fp32 = d->precision ? 1 : 0;
+  cp->decomp = d->decomp_type; cp->force_type = d->force_type; cp->dt = d->dt; cp->cap = d->force_cap;
+  cp->maxw = d->max_walkers;
+  const int L = d->L;
+  auto zero_or = [&](const double* p, std::vector<double>& v) { v.assign(L, 0.0); if (p) std::copy(p, p + L, v.begin()); };
+  std::vector<double> U, mu, hx, hy, hz, cf;
+  zero_or(d->U, U); zero_or(d->mu, mu); zero_or(d->hx, hx); zero_or(d->hy, hy); zero_or(d->hz, hz);
+  zero_or(d->const_force, cf);
+  bool mixing = false;
+  for (int i = 0; i < L; ++i) if (hx[i] != 0.0 || hy[i] != 0.0) mixing = true;
+  const bool hop = cp->decomp >= 2;
+  if (d->spin_blocks) {
+    if (hop || mixing) {
+      delete cp;
+      REQUIRE(false, "spin_blocks needs a density decomposition and hx = hy = 0 (the HS field must not mix spins)");
+    }
+    REQUIRE(d->Nup >= 0 && d->Nup <= d->N, "invalid Nup");
+    cp->nblk = 2; cp->dense = 0;
+    cp->Nup = d->Nup; cp->Ndn = d->N - d->Nup;
+    cp->Mb[0] = cp->Mb[1] = L;
+    cp->Nb[0] = cp->Nup; cp->Nb[1] = cp->Ndn;
+    cp->row0[0] = 0; cp->row0[1] = L; cp->col0[0] = 0; cp->col0[1] = cp->Nup;
+    if (cp->Nup == 0 || cp->Ndn == 0) {
+      delete cp;
+      REQUIRE(false, "spin_blocks needs Nup > 0 and Ndn > 0");
+    }
+  } else {
+    cp->nblk = 1; cp->dense = 1;
+    cp->Mb[0] = 2 * L; cp->Nb[0] = d->N;
+  }
+  for (int b = 0; b < cp->nblk; ++b) {
+    if (cp->Nb[b] > 160) {
+      delete cp;
+      REQUIRE(false, "block column count %d exceeds the supported 160", 0);
+    }
+  }
+  cp->diag_only = (!hop && !mixing) ? 1 : 0;
+  // NiupNidn constants (NiupNidn.cpp:12-26, 201-250)
+  cp->h_gamma.resize(L);
+  cp->h_cdiag.resize(4 * L);
+  double usum = 0.0;
+  for (int i = 0; i < L; ++i) usum += U[i];
+  cp->dtUSum = d->dt * usum;
+  cp->log_half_L = std::log(0.5) * L;
+  for (int i = 0; i < L; ++i) {
+    double dtU = d->dt * U[i];
+    switch (cp->decomp) {
+      case 0: cp->h_gamma[i] = solve_cosh_exp(-dtU * 0.5); break;
+      case 1: case 2: cp->h_gamma[i] = solve_cosh_exp(dtU * 0.5); break;
+      default: cp->h_gamma[i] = solve_cos_exp(dtU * 0.5); break;
+    }
+    double a = -d->dt * (U[i] * 0.5 - mu[i] + hz[i] * 0.5);
+    double b = -d->dt * (U[i] * 0.5 - mu[i] - hz[i] * 0.5);
+    cd c = -d->dt * 0.5 * cd(hx[i], hy[i]);
+    exp2by2(a, b, c);
+    cp->h_cdiag[i] = a; cp->h_cdiag[L + i] = std::conj(c); cp->h_cdiag[2 * L + i] = c; cp->h_cdiag[3 * L + i] = b;
+  }
+  int rc = 0;
+  auto up = [&](DevBuf<double>& dst, const std::vector<double>& v) {
+    if (rc) return;
+    rc = dst.alloc(v.size());
+    if (!rc && hipMemcpy(dst.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+  };
+  up(cp->U, U); up(cp->mu, mu); up(cp->hx, hx); up(cp->hy, hy); up(cp->hz, hz); up(cp->cforce, cf);
+  if (!rc) rc = cp->gamma.alloc(L);
+  if (!rc) rc = cp->cdiag.alloc(4 * L);
+  if (!rc && hipMemcpy(cp->gamma.p, cp->h_gamma.data(), sizeof(zd) * L, hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+  if (!rc && hipMemcpy(cp->cdiag.p, cp->h_cdiag.data(), sizeof(zd) * 4 * L, hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+  // trial, propagators, (phiT^H K)^T
+  const int L2 = 2 * L;
+  std::vector<cd> K((const cd*)d->K, (const cd*)d->K + (size_t)L2 * L2);
+  std::vector<cd> phi((const cd*)d->phiT, (const cd*)d->phiT + (size_t)L2 * d->N);
+  std::vector<cd> wdkT((size_t)L2 * d->N);  // (phiT^H K)^T = K^T conj(phiT): 2L x N
+  for (int n = 0; n < d->N; ++n)
+    for (int r = 0; r < L2; ++r) {
+      cd s = 0.0;
+      for (int q = 0; q < L2; ++q) s += std::conj(phi[q + (size_t)n * L2]) * K[q + (size_t)r * L2];
+      wdkT[r + (size_t)n * L2] = s;
+    }
+  const double* mats[3] = {d->exp_minus_dt_k, d->exp_minus_half_dt_k, d->exp_half_dt_k};
+  for (int k = 0; k < 3; ++k) cp->hop_logw[k] = cmk<double>(d->hop_logw[2 * k], d->hop_logw[2 * k + 1]);
+  cp->trial_logw = cmk<double>(d->phiT_logw[0], d->phiT_logw[1]);
+  if (!rc) {
+    if (cp->fp32) {
+      rc = upload_matrix_blocks<float>(cp, d->phiT, d->N, cp->phiT, false);
+      for (int k = 0; k < 3 && !rc; ++k) rc = upload_matrix_blocks<float>(cp, mats[k], L2, cp->expK[k], true);
+      if (!rc) rc = upload_matrix_blocks<float>(cp, (const double*)wdkT.data(), d->N, cp->wdkT, false);
+    } else {
+      rc = upload_matrix_blocks<double>(cp, d->phiT, d->N, cp->phiT, false);
+      for (int k = 0; k < 3 && !rc; ++k) rc = upload_matrix_blocks<double>(cp, mats[k], L2, cp->expK[k], true);
+      if (!rc) rc = upload_matrix_blocks<double>(cp, (const double*)wdkT.data(), d->N, cp->wdkT, false);
+    }
+  }
+  // workspace
+  const int nw = d->max_walkers;
+  const size_t es = cp->esz();
+  for (int b = 0; b < cp->nblk && !rc; ++b) {
+    const long long Mb = cp->Mb[b], Nb = cp->Nb[b];
+    cp->ntile[b] = (int)((Nb + 63) / 64);
+    rc = cp->ovl[b].alloc_bytes(es * Nb * Nb * nw);
+    if (!rc) rc = cp->ldet[b].alloc(nw);
+    if (!rc) rc = cp->info[b].alloc(nw);
+    for (int g = 0; g < 3 && !rc; ++g) rc = cp->gpart[g][b].alloc((size_t)nw * cp->ntile[b] * Mb);
+    if (!rc) rc = cp->scale[b].alloc_bytes(es * Mb * nw);
+    if (!rc) rc = cp->qrS[b].alloc_bytes(sizeof(zd) * Nb * Nb * nw);
+    if (!rc) rc = cp->qrld[b].alloc(nw);
+    if (!rc) rc = cp->qrinfo[b].alloc(nw);
+    if (!rc && cp->fp32) rc = cp->qr64[b].alloc_bytes(sizeof(zd) * Mb * Nb * nw);
+    if (!rc && cp->fp32) rc = cp->qr64b[b].alloc_bytes(sizeof(zd) * Mb * Nb * nw);
+  }
+  if (!rc && !cp->diag_only) rc = cp->d2x2.alloc_bytes(es * 4 * L * nw);
+  if (!rc) rc = cp->logO.alloc(nw);
+  if (!rc) rc = cp->den.alloc(nw);
+  if (!rc) rc = cp->prefix.alloc(nw);
+  if (!rc) rc = cp->table.alloc(nw);
+  if (!rc) rc = cp->comb_num.alloc(nw);
+  if (!rc) rc = cp->comb_prob.alloc(nw);
+  if (!rc) rc = cp->weight.alloc(nw);
+  if (!rc) rc = cp->per_walker.alloc((size_t)nw * 8);
+  if (!rc) rc = cp->site_acc.alloc((size_t)4 * L);
+  if (!rc) rc = cp->force_rec.alloc((size_t)nw * L);
+  if (!rc) rc = cp->aux_rec.alloc((size_t)nw * L);
+  if (!rc) rc = cp->sc.alloc(1);
+  if (!rc) {
+    DevScalars h;
+    memset(&h, 0, sizeof(h));
+    h.ET = -10.0;
+    h.key = afqmc_seed_mix_host(985456376ull);
+    if (hipMemcpy(cp->sc.p, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+    if (!rc && hipMemset(cp->site_acc.p, 0, sizeof(zd) * 4 * L) != hipSuccess) rc = -2;
+  }
+  if (rc) {
+    delete cp;
+    if (rc == -2) afqmc_set_error("afqmc_cp_create: device allocation or copy failed");
+    return rc;
+  }
+  *out = cp;
+  return 0;
+}
+
+int afqmc_cp_destroy(afqmc_cp cp) {
+  if (!cp) return 0;
+  ScopedDevice sd(cp->device);
+  hipDeviceSynchronize();
+  delete cp;
+  return 0;
+}
+
+int afqmc_cp_get_hs_constants(afqmc_cp cp, double* out) {
+  REQUIRE(cp && out, "null argument");
+  const int L = cp->L;
+  for (int i = 0; i < L; ++i) { out[2 * i] = cp->h_gamma[i].real(); out[2 * i + 1] = cp->h_gamma[i].imag(); }
+  for (int i = 0; i < 4 * L; ++i) {
+    out[2 * (L + i)] = cp->h_cdiag[i].real();
+    out[2 * (L + i) + 1] = cp->h_cdiag[i].imag();
+  }
+  return 0;
+}
+
+int afqmc_cp_walkers_create(afqmc_cp cp, int nw, afqmc_walkers* out) {
+  REQUIRE(cp && out, "null argument");
+  REQUIRE(nw > 0 && nw <= cp->maxw, "walker count %d outside 1..max_walkers=%d", nw, cp->maxw);
+  ScopedDevice sd(cp->ctx);
+  afqmc_walkers w = new afqmc_walkers_s();
+  w->cp = cp;
+  w->device = cp->device;
+  w->nw = nw;
+  int rc = 0;
+  for (int b = 0; b < cp->nblk && !rc; ++b) {
+    rc = w->wf[b].alloc_bytes(cp->esz() * w->wsize(b) * nw);
+    if (!rc) rc = w->tmp[b].alloc_bytes(cp->esz() * w->wsize(b) * nw);
+    if (!rc && hipMemset(w->wf[b].p, 0, cp->esz() * w->wsize(b) * nw) != hipSuccess) rc = -2;
+  }
+  if (!rc) rc = w->logw.alloc(nw);
+  if (!rc) rc = w->logw_tmp.alloc(nw);
+  if (!rc) rc = w->alive.alloc(nw);
+  if (!rc) {
+    std::vector<int> one(nw, 1);
+    std::vector<zd> z(nw, cmk<double>(0, 0));
+    if (hipMemcpy(w->alive.p, one.data(), sizeof(int) * nw, hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+    if (!rc && hipMemcpy(w->logw.p, z.data(), sizeof(zd) * nw, hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+  }
+  if (rc) {
+    delete w;
+    if (rc == -2) afqmc_set_error("afqmc_cp_walkers_create: device allocation failed");
+    return rc;
+  }
+  *out = w;
+  return 0;
+}
+
+int afqmc_walkers_destroy(afqmc_walkers w) {
+  if (!w) return 0;
+  ScopedDevice sd(w->device);
+  hipDeviceSynchronize();
+  delete w;
+  return 0;
+}
+
+int afqmc_walkers_count(afqmc_walkers w) { return w ? w->nw : 0; }
+
+int afqmc_walkers_upload(afqmc_walkers w, const double* wf, const double* logw, const int* alive) {
+  REQUIRE(w && wf && logw, "null argument");
+  afqmc_cp cp = w->cp;
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  const int L2 = 2 * cp->L, N = cp->N;
+  const long long per = (long long)L2 * N;
+  if (!cp->dense) {
+    // the off-block entries must be zero for the block representation to be exact
+    for (int iw = 0; iw < w->nw; ++iw)
+      for (int j = 0; j < N; ++j)
+        for (int i = 0; i < L2; ++i) {
+          bool in_up = i < cp->L && j < cp->Nup;
+          bool in_dn = i >= cp->L && j >= cp->Nup;
+          if (!in_up && !in_dn) {
+            const double* z = wf + 2 * (iw * per + i + (long long)j * L2);
+            REQUIRE(z[0] == 0.0 && z[1] == 0.0, "walker %d is not block diagonal (row %d, column %d)", iw, i, j);
+          }
+        }
+  }
+  for (int b = 0; b < cp->nblk; ++b) {
+    const long long ws = w->wsize(b);
+    if (cp->fp32) {
+      std::vector<zf> h(ws * w->nw);
+      for (int iw = 0; iw < w->nw; ++iw)
+        pack_block<float>(wf + 2 * iw * per, L2, N, cp->row0[b], cp->col0[b], cp->Mb[b], cp->Nb[b], h, iw * ws);
+      HIP_TRY(hipMemcpy(w->wf[b].p, h.data(), sizeof(zf) * h.size(), hipMemcpyHostToDevice));
+    } else {
+      std::vector<zd> h(ws * w->nw);
+      for (int iw = 0; iw < w->nw; ++iw)
+        pack_block<double>(wf + 2 * iw * per, L2, N, cp->row0[b], cp->col0[b], cp->Mb[b], cp->Nb[b], h, iw * ws);
+      HIP_TRY(hipMemcpy(w->wf[b].p, h.data(), sizeof(zd) * h.size(), hipMemcpyHostToDevice));
+    }
+  }
+  HIP_TRY(hipMemcpy(w->logw.p, logw, sizeof(zd) * w->nw, hipMemcpyHostToDevice));
+  std::vector<int> a(w->nw, 1);
+  if (alive) std::copy(alive, alive + w->nw, a.begin());
+  HIP_TRY(hipMemcpy(w->alive.p, a.data(), sizeof(int) * w->nw, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int afqmc_walkers_download(afqmc_walkers w, double* wf, double* logw, int* alive) {
+  REQUIRE(w, "null argument");
+  afqmc_cp cp = w->cp;
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  const int L2 = 2 * cp->L, N = cp->N;
+  const long long per = (long long)L2 * N;
+  if (wf) {
+    std::fill(wf, wf + 2 * per * w->nw, 0.0);
+    for (int b = 0; b < cp->nblk; ++b) {
+      const long long ws = w->wsize(b);
+      std::vector<zd> h(ws * w->nw);
+      if (cp->fp32) {
+        std::vector<zf> f(ws * w->nw);
+        HIP_TRY(hipMemcpy(f.data(), w->wf[b].p, sizeof(zf) * f.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < f.size(); ++k) h[k] = cmk<double>(f[k].re, f[k].im);
+      } else {
+        HIP_TRY(hipMemcpy(h.data(), w->wf[b].p, sizeof(zd) * h.size(), hipMemcpyDeviceToHost));
+      }
+      for (int iw = 0; iw < w->nw; ++iw)
+        for (int j = 0; j < cp->Nb[b]; ++j)
+          for (int i = 0; i < cp->Mb[b]; ++i) {
+            zd v = h[iw * ws + i + (long long)j * cp->Mb[b]];
+            double* dst = wf + 2 * (iw * per + (cp->row0[b] + i) + (long long)(cp->col0[b] + j) * L2);
+            dst[0] = v.re;
+            dst[1] = v.im;
+          }
+    }
+  }
+  if (logw) HIP_TRY(hipMemcpy(logw, w->logw.p, sizeof(zd) * w->nw, hipMemcpyDeviceToHost));
+  if (alive) HIP_TRY(hipMemcpy(alive, w->alive.p, sizeof(int) * w->nw, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int afqmc_cp_set_seed(afqmc_cp cp, uint64_t seed, uint64_t counter) {
+  REQUIRE(cp, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  DevScalars h;
+  HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
+  h.key = afqmc_seed_mix_host(seed);
+  h.counter = counter;
+  HIP_TRY(hipMemcpy(cp->sc.p, &h, sizeof(h), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int afqmc_cp_get_counter(afqmc_cp cp, uint64_t* counter) {
+  REQUIRE(cp && counter, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  DevScalars h;
+  HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
+  *counter = h.counter;
+  return 0;
+}
+
+int afqmc_cp_set_uniforms(afqmc_cp cp, const double* u, int64_t n) {
+  REQUIRE(cp && n >= 0, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  cp->ubuf.release();
+  if (n > 0) {
+    REQUIRE(u, "null uniforms");
+    int rc = cp->ubuf.alloc((size_t)n);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(cp->ubuf.p, u, sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  // host-provided uniforms are indexed from 0
+  DevScalars h;
+  HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
+  h.counter = 0;
+  HIP_TRY(hipMemcpy(cp->sc.p, &h, sizeof(h), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int afqmc_cp_skip_uniforms(afqmc_cp cp, uint64_t n) {
+  REQUIRE(cp, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  unsigned long long c = 0;
+  HIP_TRY(hipMemcpy(&c, (char*)cp->sc.p + offsetof(DevScalars, counter), sizeof(c), hipMemcpyDeviceToHost));
+  c += n;
+  HIP_TRY(hipMemcpy((char*)cp->sc.p + offsetof(DevScalars, counter), &c, sizeof(c), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int afqmc_cp_set_ET(afqmc_cp cp, double ET) {
+  REQUIRE(cp, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  HIP_TRY(hipMemcpy((char*)cp->sc.p + offsetof(DevScalars, ET), &ET, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int afqmc_cp_get_ET(afqmc_cp cp, double* ET) {
+  REQUIRE(cp && ET, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  HIP_TRY(hipMemcpy(ET, (char*)cp->sc.p + offsetof(DevScalars, ET), sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int afqmc_cp_log_overlap(afqmc_cp cp, afqmc_walkers w, double* logO) {
+  REQUIRE(cp && w && logO && w->cp == cp, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  int rc = cp->fp32 ? do_overlap<float>(cp, w, false) : do_overlap<double>(cp, w, false);
+  if (rc) return rc;
+  LogOArgs l = cp->fp32 ? logo_args<float>(cp, w, LOGO_ONLY) : logo_args<double>(cp, w, LOGO_ONLY);
+  if ((rc = launch_logo(l, cp->ctx->stream))) return rc;
+  HIP_TRY(hipMemcpyAsync(logO, cp->logO.p, sizeof(zd) * w->nw, hipMemcpyDeviceToHost, cp->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  return 0;
+}
+
+int afqmc_cp_green_diagonal(afqmc_cp cp, afqmc_walkers w, double* gdiag, double* goff) {
+  REQUIRE(cp && w && gdiag && w->cp == cp, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  int rc = cp->fp32 ? do_overlap<float>(cp, w, true) : do_overlap<double>(cp, w, true);
+  if (!rc) rc = cp->fp32 ? green_gemm<float>(cp, w, w->wf, w->alive.p, false)
+                         : green_gemm<double>(cp, w, w->wf, w->alive.p, false);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  const int L = cp->L, L2 = 2 * L;
+  for (int b = 0; b < cp->nblk; ++b) {
+    const size_t n = (size_t)w->nw * cp->ntile[b] * cp->Mb[b];
+    std::vector<zd> g0(n), g1;
+    HIP_TRY(hipMemcpy(g0.data(), cp->gpart[0][b].p, sizeof(zd) * n, hipMemcpyDeviceToHost));
+    if (cp->dense) {
+      g1.resize(n);
+      HIP_TRY(hipMemcpy(g1.data(), cp->gpart[1][b].p, sizeof(zd) * n, hipMemcpyDeviceToHost));
+    }
+    for (int iw = 0; iw < w->nw; ++iw)
+      for (int r = 0; r < cp->Mb[b]; ++r) {
+        zd s0 = cmk<double>(0, 0), s1 = s0;
+        for (int t = 0; t < cp->ntile[b]; ++t) {
+          size_t k = ((size_t)iw * cp->ntile[b] + t) * cp->Mb[b] + r;
+          s0 = cadd(s0, g0[k]);
+          if (cp->dense) s1 = cadd(s1, g1[k]);
+        }
+        const int row = cp->row0[b] + r;
+        gdiag[2 * ((size_t)iw * L2 + row)] = s0.re;
+        gdiag[2 * ((size_t)iw * L2 + row) + 1] = s0.im;
+        if (goff) {
+          // X row r pairs with trial row r^: G(r^, r) = greenOffDiagonal(r^)
+          const int dst = cp->dense ? (row < L ? row + L : row - L) : row;
+          goff[2 * ((size_t)iw * L2 + dst)] = cp->dense ? s1.re : 0.0;
+          goff[2 * ((size_t)iw * L2 + dst) + 1] = cp->dense ? s1.im : 0.0;
+        }
+      }
+  }
+  return 0;
+}
+
+int afqmc_cp_force(afqmc_cp cp, afqmc_walkers w, double* force) {
+  REQUIRE(cp && w && force && w->cp == cp, "invalid argument");
+  const int L = cp->L, L2 = 2 * L;
+  std::vector<double> gd((size_t)w->nw * L2 * 2), go((size_t)w->nw * L2 * 2);
+  int rc = afqmc_cp_green_diagonal(cp, w, gd.data(), go.data());
+  if (rc) return rc;
+  for (int iw = 0; iw < w->nw; ++iw)
+    for (int i = 0; i < L; ++i) {
+      auto G = [&](const std::vector<double>& v, int r) { return cd(v[2 * ((size_t)iw * L2 + r)], v[2 * ((size_t)iw * L2 + r) + 1]); };
+      cd bg;
+      switch (cp->decomp) {
+        case 0: bg = G(gd, i) + G(gd, i + L) - 1.0; break;
+        case 1: bg = G(gd, i) - G(gd, i + L); break;
+        case 2: bg = G(go, i) + G(go, i + L); break;
+        default: bg = G(go, i) - G(go, i + L); break;
+      }
+      double f = (cp->h_gamma[i] * bg).real();
+      f = std::min(cp->cap, std::max(-cp->cap, f));
+      force[(size_t)iw * L + i] = f;
+    }
+  return 0;
+}
+
+int afqmc_cp_onebody(afqmc_cp cp, afqmc_walkers w, int which) {
+  REQUIRE(cp && w && w->cp == cp, "invalid argument");
+  REQUIRE(which >= 0 && which <= 2, "which must be 0, 1 or 2");
+  ScopedDevice sd(cp->ctx);
+  int rc = cp->fp32 ? propagate_gemm<float>(cp, w, which, w->wf, w->tmp, false)
+                    : propagate_gemm<double>(cp, w, which, w->wf, w->tmp, false);
+  if (rc) return rc;
+  // alive walkers take the product, killed ones keep their state (the reference skips them)
+  for (int b = 0; b < cp->nblk; ++b) {
+    if ((rc = launch_select_alive(w->nw, w->wsize(b), cp->esz(), w->alive.p, w->tmp[b].p, w->wf[b].p, cp->ctx->stream)))
+      return rc;
+  }
+  if ((rc = launch_logw_const_add(w->nw, w->logw.p, w->alive.p, cp->hop_logw[which], cp->ctx->stream))) return rc;
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  return 0;
+}
+
+int afqmc_cp_hs_apply(afqmc_cp cp, afqmc_walkers w, const double* diag4, const double* sample_logw) {
+  REQUIRE(cp && w && diag4 && sample_logw && w->cp == cp, "invalid argument");
+  REQUIRE(cp->dense, "afqmc_cp_hs_apply needs a dense (spin_blocks = 0) set");
+  ScopedDevice sd(cp->ctx);
+  hipStream_t s = cp->ctx->stream;
+  const int L = cp->L;
+  const size_t n = (size_t)4 * L * w->nw;
+  int rc = 0;
+  DevBuf<char> d;
+  if ((rc = d.alloc_bytes(cp->esz() * n))) return rc;
+  if (cp->fp32) {
+    std::vector<zf> h(n);
+    for (size_t k = 0; k < n; ++k) h[k] = cmk<float>((float)diag4[2 * k], (float)diag4[2 * k + 1]);
+    HIP_TRY(hipMemcpy(d.p, h.data(), sizeof(zf) * n, hipMemcpyHostToDevice));
+    rc = launch_hs_apply_2x2<float>(w->nw, L, cp->N, w->wf[0].p, w->tmp[0].p, d.p, s);
+  } else {
+    HIP_TRY(hipMemcpy(d.p, diag4, sizeof(zd) * n, hipMemcpyHostToDevice));
+    rc = launch_hs_apply_2x2<double>(w->nw, L, cp->N, w->wf[0].p, w->tmp[0].p, d.p, s);
+  }
+  if (rc) return rc;
+  if ((rc = launch_select_alive(w->nw, w->wsize(0), cp->esz(), w->alive.p, w->tmp[0].p, w->wf[0].p, s))) return rc;
+  DevBuf<zd> lw;
+  if ((rc = lw.alloc(w->nw))) return rc;
+  HIP_TRY(hipMemcpy(lw.p, sample_logw, sizeof(zd) * w->nw, hipMemcpyHostToDevice));
+  if ((rc = launch_logw_vec_add(w->nw, w->logw.p, w->alive.p, lw.p, s))) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int afqmc_cp_stabilize(afqmc_cp cp, afqmc_walkers w) {
+  REQUIRE(cp && w && w->cp == cp, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  int rc = stabilize_impl(cp, w);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  return 0;
+}
+
+int afqmc_comb(afqmc_ctx ctx, int nw, const double* weight, const double* uniforms, int* table, double* stats) {
+  REQUIRE(ctx && weight && uniforms && table && nw > 0, "invalid argument");
+  ScopedDevice sd(ctx);
+  hipStream_t s = ctx->stream;
+  DevBuf<double> w, u, prob;
+  DevBuf<int> t, num;
+  DevBuf<DevScalars> sc;
+  int rc = w.alloc(nw);
+  if (!rc) rc = u.alloc(nw);
+  if (!rc) rc = prob.alloc(nw);
+  if (!rc) rc = t.alloc(nw);
+  if (!rc) rc = num.alloc(nw);
+  if (!rc) rc = sc.alloc(1);
+  if (rc) return rc;
+  DevScalars h;
+  memset(&h, 0, sizeof(h));
+  HIP_TRY(hipMemcpy(sc.p, &h, sizeof(h), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.p, weight, sizeof(double) * nw, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(u.p, uniforms, sizeof(double) * nw, hipMemcpyHostToDevice));
+  if ((rc = launch_comb(nw, w.p, u.p, sc.p, t.p, prob.p, num.p, s))) return rc;
+  HIP_TRY(hipMemcpyAsync(table, t.p, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&h, sc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (stats) { stats[0] = h.wmin; stats[1] = h.wmax; stats[2] = h.wavg; }
+  return 0;
+}
+
+int afqmc_cp_apply_table(afqmc_cp cp, afqmc_walkers w, const int* table) {
+  REQUIRE(cp && w && table && w->cp == cp, "invalid argument");
+  for (int i = 0; i < w->nw; ++i) REQUIRE(table[i] >= 0 && table[i] < w->nw, "table entry %d out of range", i);
+  ScopedDevice sd(cp->ctx);
+  hipStream_t s = cp->ctx->stream;
+  HIP_TRY(hipMemcpy(cp->table.p, table, sizeof(int) * w->nw, hipMemcpyHostToDevice));
+  void* wfp[2] = {w->wf[0].p, cp->nblk > 1 ? w->wf[1].p : nullptr};
+  long long ws[2] = {w->wsize(0), cp->nblk > 1 ? w->wsize(1) : 0};
+  int rc = cp->fp32 ? launch_apply_table<float>(w->nw, cp->nblk, wfp, ws, w->logw.p, w->alive.p, cp->table.p, s)
+                    : launch_apply_table<double>(w->nw, cp->nblk, wfp, ws, w->logw.p, w->alive.p, cp->table.p, s);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int afqmc_cp_step(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out) {
+  REQUIRE(cp && w && w->cp == cp, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  return DISPATCH(cp, step_impl, cp, w, force_out, aux_out);
+}
+
+int afqmc_cp_pop_control(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats) {
+  REQUIRE(cp && w && w->cp == cp, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  return DISPATCH(cp, pop_control_impl, cp, w, weights_out, table_out, stats);
+}
+
+int afqmc_cp_measure(afqmc_cp cp, afqmc_walkers w) {
+  REQUIRE(cp && w && w->cp == cp, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  return DISPATCH(cp, measure_impl, cp, w);
+}
+
+int afqmc_cp_energy(afqmc_cp cp, double* out) {
+  REQUIRE(cp && out, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  DevScalars h;
+  HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
+  zd v[7] = {h.den, h.HNum, h.KNum, h.VNum, h.RNum, h.NupTot, h.NdnTot};
+  for (int k = 0; k < 7; ++k) { out[2 * k] = v[k].re; out[2 * k + 1] = v[k].im; }
+  return 0;
+}
+
+int afqmc_cp_site_measure(afqmc_cp cp, double* out) {
+  REQUIRE(cp && out, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  HIP_TRY(hipMemcpy(out, cp->site_acc.p, sizeof(zd) * 4 * cp->L, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int afqmc_cp_reset_measure(afqmc_cp cp) {
+  REQUIRE(cp, "null argument");
+  ScopedDevice sd(cp->ctx);
+  int rc = launch_reset_measure(cp->sc.p, cp->site_acc.p, cp->L, cp->ctx->stream);
+  if (rc) return rc;
+  return 0;
+}
+
+int afqmc_cp_adjust_ET(afqmc_cp cp) {
+  REQUIRE(cp, "null argument");
+  ScopedDevice sd(cp->ctx);
+  int rc = launch_set_et_from_energy(cp->sc.p, cp->ctx->stream);
+  if (rc) return rc;
+  return launch_reset_measure(cp->sc.p, cp->site_acc.p, cp->L, cp->ctx->stream);
+}
+
+}  // extern "C"

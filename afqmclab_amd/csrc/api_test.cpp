@@ -1,0 +1,119 @@
+// Test hooks: run one kernel family on host data (used by the GPU unit tests
+// that check MFMA operand layouts, the Gauss-Jordan sweep and CholeskyQR).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "gemm.hpp"
+#include "runtime.hpp"
+#include "small_dense.hpp"
+
+namespace {
+
+template <typename T>
+std::vector<cplx<T>> narrow(const double* p, size_t n) {
+  std::vector<cplx<T>> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = cmk<T>((T)p[2 * i], (T)p[2 * i + 1]);
+  return v;
+}
+
+template <typename T>
+int zgemm_t(afqmc_ctx ctx, int opa, int M, int N, int K, int batch, const double* A, long long strideA, const double* B,
+            const double* scale, int scale_group, double* C) {
+  hipStream_t s = ctx->stream;
+  const long long lda = (opa == OP_N) ? M : K;
+  const size_t nA = (size_t)(strideA ? strideA * batch : (long long)M * K);
+  const size_t nB = (size_t)K * N * batch, nC = (size_t)M * N * batch;
+  auto hA = narrow<T>(A, nA), hB = narrow<T>(B, nB);
+  DevBuf<cplx<T>> dA, dB, dC, dS;
+  int rc = dA.alloc(nA);
+  if (!rc) rc = dB.alloc(nB);
+  if (!rc) rc = dC.alloc(nC);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(dA.p, hA.data(), sizeof(cplx<T>) * nA, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dB.p, hB.data(), sizeof(cplx<T>) * nB, hipMemcpyHostToDevice));
+  GemmArgs a;
+  memset(&a, 0, sizeof(a));
+  a.M = M; a.N = N; a.K = K; a.batch = batch;
+  a.A = dA.p; a.lda = lda; a.strideA = strideA;
+  a.B = dB.p; a.ldb = K; a.strideB = (long long)K * N;
+  a.C = dC.p; a.ldc = M; a.strideC = (long long)M * N;
+  a.alpha_re = 1.0;
+  a.scale_group = 1;
+  if (scale) {
+    // one K-vector per column group of scale_group columns (per batch: N / scale_group groups)
+    const int groups = (N + scale_group - 1) / scale_group;
+    const size_t nS = (size_t)K * groups * batch;
+    auto hS = narrow<T>(scale, nS);
+    if ((rc = dS.alloc(nS))) return rc;
+    HIP_TRY(hipMemcpy(dS.p, hS.data(), sizeof(cplx<T>) * nS, hipMemcpyHostToDevice));
+    a.scale = dS.p; a.scale_group = scale_group; a.scale_strideG = K; a.scale_strideB = (long long)K * groups;
+  }
+  if ((rc = launch_cgemm<T>(opa, EPI_STORE, a, s))) return rc;
+  std::vector<cplx<T>> hC(nC);
+  HIP_TRY(hipMemcpyAsync(hC.data(), dC.p, sizeof(cplx<T>) * nC, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (size_t i = 0; i < nC; ++i) { C[2 * i] = hC[i].re; C[2 * i + 1] = hC[i].im; }
+  return 0;
+}
+
+template <typename T>
+int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ainv, double* logdet, int* info) {
+  hipStream_t s = ctx->stream;
+  const size_t n = (size_t)N * N * batch;
+  auto h = narrow<T>(A, n);
+  DevBuf<cplx<T>> d;
+  DevBuf<zd> ld;
+  DevBuf<int> inf;
+  int rc = d.alloc(n);
+  if (!rc) rc = ld.alloc(batch);
+  if (!rc) rc = inf.alloc(batch);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(d.p, h.data(), sizeof(cplx<T>) * n, hipMemcpyHostToDevice));
+  SmallDenseArgs a;
+  memset(&a, 0, sizeof(a));
+  a.N = N; a.batch = batch; a.in = d.p; a.ld_in = N; a.stride_in = (long long)N * N;
+  a.out = d.p; a.ld_out = N; a.stride_out = a.stride_in; a.logdet = ld.p; a.info = inf.p;
+  a.write_inverse = 1; a.nsub = N; a.alive_div = 1;
+  rc = kind == 0 ? launch_gj_inverse<T>(a, s) : launch_chol_inverse<T>(a, s);
+  if (rc) return rc;
+  std::vector<cplx<T>> o(n);
+  HIP_TRY(hipMemcpyAsync(o.data(), d.p, sizeof(cplx<T>) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(logdet, ld.p, sizeof(zd) * batch, hipMemcpyDeviceToHost, s));
+  if (info) HIP_TRY(hipMemcpyAsync(info, inf.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; ++i) { Ainv[2 * i] = o[i].re; Ainv[2 * i + 1] = o[i].im; }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K, int batch, const double* A,
+                     long long strideA, const double* B, const double* scale, int scale_group, double* C) {
+  REQUIRE(ctx && A && B && C, "null argument");
+  REQUIRE(M > 0 && N > 0 && K > 0 && batch > 0, "sizes must be positive");
+  REQUIRE(opa >= 0 && opa <= 2, "opa must be 0 (N), 1 (T) or 2 (C)");
+  REQUIRE(!scale || opa == OP_N, "row scaling is supported with op(A) = N");
+  ScopedDevice sd(ctx);
+  return precision ? zgemm_t<float>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, C)
+                   : zgemm_t<double>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, C);
+}
+
+int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,
+                          double* logdet, int* info) {
+  REQUIRE(ctx && A && Ainv && logdet, "null argument");
+  ScopedDevice sd(ctx);
+  return precision ? gj_t<float>(ctx, 0, N, batch, A, Ainv, logdet, info)
+                   : gj_t<double>(ctx, 0, N, batch, A, Ainv, logdet, info);
+}
+
+int afqmc_test_chol_inverse(afqmc_ctx ctx, int N, int batch, const double* S, double* Rinv, double* logsum,
+                            int* info) {
+  REQUIRE(ctx && S && Rinv && logsum, "null argument");
+  ScopedDevice sd(ctx);
+  return gj_t<double>(ctx, 1, N, batch, S, Rinv, logsum, info);
+}
+
+}  // extern "C"

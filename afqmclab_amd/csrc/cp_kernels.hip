@@ -1,0 +1,527 @@
+// Walker-level kernels of the CP step (afqmcConstraintPathProjection.cpp:37-156),
+// the HS sampling of NiupNidn (NiupNidn.cpp:66-322), the mixed energy of
+// HubbardSOCMeasureFixedSDSD (:178-254) and the comb (populationControlConfiguration.cpp:24-101).
+#include "cp_kernels.hpp"
+
+unsigned long long afqmc_seed_mix_host(unsigned long long seed) {
+  unsigned long long z = seed * 0x9E3779B97F4A7C15ull + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+__device__ double block_sum_d(double v, double* sh) {
+  const int tid = threadIdx.x;
+  v = wave_sum(v);
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  const int nwv = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nwv; ++i) t += sh[i];
+  __syncthreads();
+  return t;
+}
+__device__ zd block_sum_z(zd v, double* sh) {
+  double r = block_sum_d(v.re, sh);
+  double i = block_sum_d(v.im, sh);
+  return cmk<double>(r, i);
+}
+
+// ------------------------------------------------------------------ logO
+__global__ void logo_kernel(LogOArgs a) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= a.nw) return;
+  if (a.mode != LOGO_MEASURE && a.mode != LOGO_ONLY && !a.alive[w]) {
+    if (a.mode == LOGO_WEIGHT) a.weight[w] = 0.0;
+    return;
+  }
+  if (a.mode == LOGO_MEASURE && !a.alive[w]) {
+    a.den[w] = cmk<double>(0, 0);
+    return;
+  }
+  double re = 0.0, ang = 0.0;
+  int singular = 0;
+  if (a.combine == COMBINE_WRAP && a.nblk > 1) {
+    zd ph = cmk<double>(1.0, 0.0);
+    for (int b = 0; b < a.nblk; ++b) {
+      zd l = a.ldet[b][w];
+      re += l.re;
+      double s, c;
+      sincos(l.im, &s, &c);
+      ph = cmul(ph, cmk<double>(c, s));
+      if (a.info[b][w]) singular = 1;
+    }
+    ang = atan2(ph.im, ph.re);
+  } else {
+    for (int b = 0; b < a.nblk; ++b) {
+      zd l = a.ldet[b][w];
+      re += l.re;
+      ang += l.im;
+      if (a.info[b][w]) singular = 1;
+    }
+  }
+  if (singular) { re = -1e300; ang = 0.0; }
+  zd lw = a.logw[w];
+  zd lo = cmk<double>(a.trial_logw.re + lw.re + re, -a.trial_logw.im + lw.im + ang);
+  a.logO[w] = lo;
+  if (a.mode == LOGO_STEP) {
+    double c = cos(lo.im);
+    if (c <= 0.0) {
+      a.alive[w] = 0;
+    } else {
+      a.logw_out[w] = cmk<double>(lw.re + log(c), lw.im - lo.im);
+    }
+  } else if (a.mode == LOGO_WEIGHT) {
+    double o = (lo.re < -700.0) ? 0.0 : exp(lo.re) * cos(lo.im);
+    if (o <= 0.0) {
+      a.weight[w] = 0.0;
+      a.alive[w] = 0;
+    } else {
+      a.weight[w] = o;
+      a.logw_out[w] = cmk<double>(lw.re - lo.re, lw.im - lo.im);
+    }
+  } else if (a.mode == LOGO_MEASURE) {
+    a.den[w] = cexp_d(lo);
+  }
+}
+
+// ------------------------------------------------------------------ alive prefix
+__global__ __launch_bounds__(1024) void alive_scan_kernel(int nw, const int* alive, int* prefix, DevScalars* sc) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int chunk = (nw + 1023) / 1024;
+  const int lo = tid * chunk, hi = min(nw, lo + chunk);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += alive[i] ? 1 : 0;
+  part[tid] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = (tid >= o) ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int base = part[tid] - s;
+  for (int i = lo; i < hi; ++i) {
+    prefix[i] = base;
+    base += alive[i] ? 1 : 0;
+  }
+  if (tid == 1023) sc->alive_count = part[1023];
+}
+
+// ------------------------------------------------------------------ HS sampling
+__device__ __forceinline__ zd gsum(const zd* g, int w, int ntile, int Mb, int row) {
+  zd s = cmk<double>(0, 0);
+  const zd* p = g + (long long)w * ntile * Mb + row;
+  for (int t = 0; t < ntile; ++t) s = cadd(s, p[(long long)t * Mb]);
+  return s;
+}
+
+struct SiteG { zd gup, gdn, off_i, off_iL; };
+
+__device__ __forceinline__ SiteG site_green(const zd* const* gd, const zd* const* go, const int* ntile, const int* Mb,
+                                            int dense, int w, int i, int L) {
+  SiteG g;
+  if (dense) {
+    g.gup = gsum(gd[0], w, ntile[0], Mb[0], i);
+    g.gdn = gsum(gd[0], w, ntile[0], Mb[0], i + L);
+    if (go[0]) {
+      // d1 of X row r is G(r^, r): greenOffDiagonal(i) = G(i, i+L) sits at row i+L
+      g.off_i = gsum(go[0], w, ntile[0], Mb[0], i + L);
+      g.off_iL = gsum(go[0], w, ntile[0], Mb[0], i);
+    } else {
+      g.off_i = g.off_iL = cmk<double>(0, 0);
+    }
+  } else {
+    g.gup = gsum(gd[0], w, ntile[0], Mb[0], i);
+    g.gdn = gsum(gd[1], w, ntile[1], Mb[1], i);
+    g.off_i = g.off_iL = cmk<double>(0, 0);
+  }
+  return g;
+}
+
+__device__ __forceinline__ zd cexp_c(zd a) { return cexp_d(a); }
+__device__ __forceinline__ zd ccosh_c(zd a) {
+  zd e = cexp_d(a), em = cexp_d(cmk<double>(-a.re, -a.im));
+  return cmk<double>(0.5 * (e.re + em.re), 0.5 * (e.im + em.im));
+}
+__device__ __forceinline__ zd csinh_c(zd a) {
+  zd e = cexp_d(a), em = cexp_d(cmk<double>(-a.re, -a.im));
+  return cmk<double>(0.5 * (e.re - em.re), 0.5 * (e.im - em.im));
+}
+__device__ __forceinline__ zd ccos_c(zd a) { return ccosh_c(cmk<double>(-a.im, a.re)); }      // cos z = cosh(iz)
+__device__ __forceinline__ zd csin_c(zd a) {                                                     // sin z = -i sinh(iz)
+  zd s = csinh_c(cmk<double>(-a.im, a.re));
+  return cmk<double>(s.im, -s.re);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
+  const int w = blockIdx.x;
+  if (!a.alive[w]) return;
+  __shared__ double sh[16];
+  const int L = a.L;
+  const unsigned long long base = a.sc->counter + (unsigned long long)a.prefix[w] * (unsigned long long)L;
+  const unsigned long long key = a.sc->key;
+  double lp = 0.0;
+  zd xg = cmk<double>(0, 0);
+  typedef cplx<T> ct;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    SiteG g = site_green(a.gd, a.go, a.ntile, a.Mb, a.dense, w, i, L);
+    double f;
+    if (a.force_type == 1) {
+      f = a.const_force[i];
+    } else {
+      zd bg;
+      switch (a.decomp) {
+        case DECOMP_DENSITY_CHARGE: bg = cmk<double>(g.gup.re + g.gdn.re - 1.0, g.gup.im + g.gdn.im); break;
+        case DECOMP_DENSITY_SPIN: bg = csub(g.gup, g.gdn); break;
+        case DECOMP_HOP_CHARGE: bg = cadd(g.off_i, g.off_iL); break;
+        default: bg = csub(g.off_i, g.off_iL); break;
+      }
+      f = cmul(a.gamma[i], bg).re;
+      if (f > a.cap) f = a.cap;
+      if (f < -a.cap) f = -a.cap;
+    }
+    const unsigned long long k = base + (unsigned long long)i;
+    const double u = a.ubuf ? a.ubuf[k] : afqmc_uniform_dev(key, k);
+    const double em = exp(-f), ep = exp(f);
+    const double prob = em / (em + ep);
+    const int x = (u < prob) ? -1 : 1;
+    lp += x * f - log(ep + em);
+    const zd gx = cscale(a.gamma[i], (double)x);
+    xg = cadd(xg, gx);
+    const zd c00 = a.cdiag[i], c01 = a.cdiag[L + i], c10 = a.cdiag[2 * L + i], c11 = a.cdiag[3 * L + i];
+    zd d00, d01, d10, d11;
+    switch (a.decomp) {
+      case DECOMP_DENSITY_CHARGE: {
+        zd e = cexp_c(gx);
+        d00 = cmul(e, c00); d10 = cmul(e, c10); d01 = cmul(e, c01); d11 = cmul(e, c11);
+        break;
+      }
+      case DECOMP_DENSITY_SPIN: {
+        zd e = cexp_c(gx), einv = cinv(e);
+        d00 = cmul(e, c00); d10 = cmul(einv, c10); d01 = cmul(e, c01); d11 = cmul(einv, c11);
+        break;
+      }
+      case DECOMP_HOP_CHARGE: {
+        zd ch = ccosh_c(gx), sn = csinh_c(gx);
+        d00 = cadd(cmul(ch, c00), cmul(sn, c10)); d10 = cadd(cmul(sn, c00), cmul(ch, c10));
+        d01 = cadd(cmul(ch, c01), cmul(sn, c11)); d11 = cadd(cmul(sn, c01), cmul(ch, c11));
+        break;
+      }
+      default: {
+        zd co = ccos_c(gx), si = csin_c(gx);
+        d00 = cadd(cmul(co, c00), cmul(si, c10)); d10 = csub(cmul(co, c10), cmul(si, c00));
+        d01 = cadd(cmul(co, c01), cmul(si, c11)); d11 = csub(cmul(co, c11), cmul(si, c01));
+        break;
+      }
+    }
+    if (a.diag_only) {
+      if (a.dense) {
+        ct* s0 = reinterpret_cast<ct*>(a.scale[0]) + (long long)w * a.Mb[0];
+        s0[i] = from_zd<T>(d00);
+        s0[i + L] = from_zd<T>(d11);
+      } else {
+        reinterpret_cast<ct*>(a.scale[0])[(long long)w * a.Mb[0] + i] = from_zd<T>(d00);
+        reinterpret_cast<ct*>(a.scale[1])[(long long)w * a.Mb[1] + i] = from_zd<T>(d11);
+      }
+    } else {
+      ct* d = reinterpret_cast<ct*>(a.d2x2) + (long long)w * 4 * L;
+      d[i] = from_zd<T>(d00);
+      d[L + i] = from_zd<T>(d01);
+      d[2 * L + i] = from_zd<T>(d10);
+      d[3 * L + i] = from_zd<T>(d11);
+    }
+    if (a.force_out) a.force_out[(long long)w * L + i] = f;
+    if (a.aux_out) a.aux_out[(long long)w * L + i] = x;
+  }
+  lp = block_sum_d(lp, sh);
+  xg = block_sum_z(xg, sh);
+  if (threadIdx.x == 0) {
+    zd sl = cmk<double>(a.log_half_L, 0.0);
+    if (a.decomp == DECOMP_DENSITY_CHARGE) sl = cmk<double>(sl.re + a.dtUSum * 0.5 - xg.re, -xg.im);
+    sl.re -= lp;
+    zd lw = a.logw[w];
+    lw.re += sl.re + a.dt * a.sc->ET + a.onebody_logw.re;
+    lw.im += sl.im + a.onebody_logw.im;
+    a.logw[w] = lw;
+  }
+}
+
+__global__ void advance_counter_kernel(DevScalars* sc, int per_walker) {
+  sc->counter += (unsigned long long)per_walker * (unsigned long long)sc->alive_count;
+}
+
+// ------------------------------------------------------------------ HS 2x2 apply
+template <typename T>
+__global__ void hs_apply_2x2_kernel(int nw, int L, int N, const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,
+                                    const cplx<T>* __restrict__ d) {
+  const long long per = (long long)L * N;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per * nw) return;
+  const int w = (int)(idx / per);
+  const long long r = idx - (long long)w * per;
+  const int j = (int)(r / L), i = (int)(r - (long long)j * L);
+  const long long o = (long long)w * 2 * per + (long long)j * 2 * L + i;
+  const cplx<T>* dw = d + (long long)w * 4 * L;
+  const cplx<T> a = in[o], b = in[o + L];
+  out[o] = cadd(cmul(dw[i], a), cmul(dw[L + i], b));
+  out[o + L] = cadd(cmul(dw[2 * L + i], a), cmul(dw[3 * L + i], b));
+}
+
+// ------------------------------------------------------------------ comb
+// popConfiguration (populationControlConfiguration.cpp:24-101): sequential
+// left-to-right sums in one thread, so bin edges round exactly as on the host.
+__global__ void comb_kernel(int L, const double* weight, const double* ubuf, DevScalars* sc, int* table,
+                            double* prob, int* number) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long key = sc->key;
+  const unsigned long long c0 = sc->counter;
+  double sum = 0.0, maxW = 0.0, minW = 1e20;
+  for (int i = 0; i < L; ++i) {
+    double w = weight[i];
+    sum += w;
+    if (maxW < w) maxW = w;
+    if (minW > w) minW = w;
+  }
+  sc->wmin = minW;
+  sc->wmax = maxW;
+  sc->wavg = sum / (L * 1.0);
+  for (int i = 0; i < L; ++i) {
+    prob[i] = weight[i] / sum;
+    number[i] = 0;
+  }
+  int index_old = 0;
+  double prob_sum_old = prob[0];
+  for (int i = 0; i < L; ++i) {
+    const unsigned long long k = c0 + (unsigned long long)i;
+    const double u = ubuf ? ubuf[k] : afqmc_uniform_dev(key, k);
+    const double prob_sum_new = (i + u) / (L * 1.0);
+    while (prob_sum_new > prob_sum_old) {
+      if (index_old + 1 >= L) break;  // the reference has no guard here
+      index_old++;
+      prob_sum_old += prob[index_old];
+    }
+    number[index_old]++;
+  }
+  sc->counter = c0 + (unsigned long long)L;
+  for (int i = 0; i < L; ++i) table[i] = (number[i] != 0) ? i : -1;
+  int zero_i = 0, one_i = 0;
+  while (zero_i < L && number[zero_i] != 0) zero_i++;
+  while (one_i < L && number[one_i] <= 1) one_i++;
+  while (zero_i < L && one_i < L) {
+    table[zero_i] = one_i;
+    number[zero_i]++;
+    number[one_i]--;
+    while (zero_i < L && number[zero_i] != 0) zero_i++;
+    while (one_i < L && number[one_i] <= 1) one_i++;
+  }
+}
+
+template <typename T>
+__global__ void apply_table_kernel(int nw, cplx<T>* wf0, cplx<T>* wf1, long long ws0, long long ws1, zd* logw,
+                                   int* alive, const int* table) {
+  const int i = blockIdx.y;
+  const int src = table[i];
+  if (src != i && src >= 0) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ws0; e += (long long)gridDim.x * blockDim.x)
+      wf0[(long long)i * ws0 + e] = wf0[(long long)src * ws0 + e];
+    if (wf1)
+      for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ws1; e += (long long)gridDim.x * blockDim.x)
+        wf1[(long long)i * ws1 + e] = wf1[(long long)src * ws1 + e];
+    if (blockIdx.x == 0 && threadIdx.x == 0) logw[i] = logw[src];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) alive[i] = 1;
+}
+
+// ------------------------------------------------------------------ energies
+__global__ __launch_bounds__(256) void energy_kernel(EnergyArgs a) {
+  const int w = blockIdx.x;
+  __shared__ double sh[16];
+  zd* out = a.per_walker + (long long)w * 8;
+  if (!a.alive[w]) {
+    if (threadIdx.x < 8) out[threadIdx.x] = cmk<double>(0, 0);
+    return;
+  }
+  const int L = a.L;
+  zd V = cmk<double>(0, 0), R = V, K = V, nup = V, ndn = V;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    SiteG g = site_green(a.gd, a.go, a.ntile, a.Mb, a.dense, w, i, L);
+    zd t = csub(cmul(g.gup, g.gdn), cmul(g.off_i, g.off_iL));
+    V = cadd(V, cscale(t, a.U[i]));
+    R = cadd(R, cscale(g.gup, -a.mu[i] + a.hz[i] * 0.5));
+    R = cadd(R, cscale(g.gdn, -a.mu[i] - a.hz[i] * 0.5));
+    R = cadd(R, cmul(cmk<double>(a.hx[i] * 0.5, -a.hy[i] * 0.5), g.off_i));
+    R = cadd(R, cmul(cmk<double>(a.hx[i] * 0.5, a.hy[i] * 0.5), g.off_iL));
+    nup = cadd(nup, g.gup);
+    ndn = cadd(ndn, g.gdn);
+  }
+  for (int b = 0; b < a.nblk; ++b)
+    for (int r = threadIdx.x; r < a.Mb[b]; r += blockDim.x) K = cadd(K, gsum(a.gk[b], w, a.ntile[b], a.Mb[b], r));
+  V = block_sum_z(V, sh);
+  R = block_sum_z(R, sh);
+  K = block_sum_z(K, sh);
+  nup = block_sum_z(nup, sh);
+  ndn = block_sum_z(ndn, sh);
+  if (threadIdx.x == 0) {
+    zd d = a.den[w];
+    out[0] = d;
+    out[1] = cmul(cadd(cadd(K, V), R), d);
+    out[2] = cmul(K, d);
+    out[3] = cmul(V, d);
+    out[4] = cmul(R, d);
+    out[5] = cmul(nup, d);
+    out[6] = cmul(ndn, d);
+    out[7] = cadd(cadd(K, V), R);
+  }
+}
+
+__global__ __launch_bounds__(256) void energy_reduce_kernel(int nw, const zd* per_walker, DevScalars* sc) {
+  // fixed-order reduction over walkers: thread t sums a contiguous chunk, then the block sums in order
+  __shared__ zd part[256][7];
+  const int tid = threadIdx.x;
+  const int chunk = (nw + 255) / 256;
+  zd s[7];
+  for (int k = 0; k < 7; ++k) s[k] = cmk<double>(0, 0);
+  for (int w = tid * chunk; w < min(nw, (tid + 1) * chunk); ++w)
+    for (int k = 0; k < 7; ++k) s[k] = cadd(s[k], per_walker[(long long)w * 8 + k]);
+  for (int k = 0; k < 7; ++k) part[tid][k] = s[k];
+  __syncthreads();
+  if (tid == 0) {
+    zd t[7];
+    for (int k = 0; k < 7; ++k) t[k] = cmk<double>(0, 0);
+    for (int i = 0; i < 256; ++i)
+      for (int k = 0; k < 7; ++k) t[k] = cadd(t[k], part[i][k]);
+    sc->den = cadd(sc->den, t[0]);
+    sc->HNum = cadd(sc->HNum, t[1]);
+    sc->KNum = cadd(sc->KNum, t[2]);
+    sc->VNum = cadd(sc->VNum, t[3]);
+    sc->RNum = cadd(sc->RNum, t[4]);
+    sc->NupTot = cadd(sc->NupTot, t[5]);
+    sc->NdnTot = cadd(sc->NdnTot, t[6]);
+  }
+}
+
+__global__ void site_reduce_kernel(EnergyArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.L) return;
+  const int L = a.L;
+  zd su = cmk<double>(0, 0), sd = su, sp = su, sm = su;
+  for (int w = 0; w < a.nw; ++w) {
+    if (!a.alive[w]) continue;
+    SiteG g = site_green(a.gd, a.go, a.ntile, a.Mb, a.dense, w, i, L);
+    zd d = a.den[w];
+    su = cadd(su, cmul(g.gup, d));
+    sd = cadd(sd, cmul(g.gdn, d));
+    sp = cadd(sp, cmul(g.off_i, d));
+    sm = cadd(sm, cmul(g.off_iL, d));
+  }
+  a.site_acc[i] = cadd(a.site_acc[i], su);
+  a.site_acc[L + i] = cadd(a.site_acc[L + i], sd);
+  a.site_acc[2 * L + i] = cadd(a.site_acc[2 * L + i], sp);
+  a.site_acc[3 * L + i] = cadd(a.site_acc[3 * L + i], sm);
+}
+
+__global__ void set_et_kernel(DevScalars* sc) {
+  zd e = cdiv(sc->HNum, sc->den);
+  sc->ET = e.re;
+}
+
+__global__ void logw_add_kernel(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, double m0,
+                                double m1) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw || (alive && !alive[w])) return;
+  double d = m0 * add0[w].re + m1 * add0[w].im;
+  if (add1) d += m0 * add1[w].re + m1 * add1[w].im;
+  logw[w].re += d;
+}
+
+}  // namespace
+
+int launch_logo(const LogOArgs& a, hipStream_t s) {
+  if (a.nw <= 0) return 0;
+  hipLaunchKernelGGL(logo_kernel, dim3((a.nw + 255) / 256), dim3(256), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_alive_scan(int nw, const int* alive, int* prefix, DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(alive_scan_kernel, dim3(1), dim3(1024), 0, s, nw, alive, prefix, sc);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_hs_sample(const HsSampleArgs& a, hipStream_t s) {
+  if (a.fp32) hipLaunchKernelGGL(hs_sample_kernel<float>, dim3(a.nw), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(hs_sample_kernel<double>, dim3(a.nw), dim3(256), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_advance_counter(DevScalars* sc, int per_walker, hipStream_t s) {
+  hipLaunchKernelGGL(advance_counter_kernel, dim3(1), dim3(1), 0, s, sc, per_walker);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+int launch_hs_apply_2x2(int nw, int L, int N, const void* in, void* out, const void* d2x2, hipStream_t s) {
+  long long tot = (long long)nw * L * N;
+  if (tot == 0) return 0;
+  hipLaunchKernelGGL(hs_apply_2x2_kernel<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nw, L, N,
+                     reinterpret_cast<const cplx<T>*>(in), reinterpret_cast<cplx<T>*>(out),
+                     reinterpret_cast<const cplx<T>*>(d2x2));
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+template int launch_hs_apply_2x2<double>(int, int, int, const void*, void*, const void*, hipStream_t);
+template int launch_hs_apply_2x2<float>(int, int, int, const void*, void*, const void*, hipStream_t);
+
+int launch_comb(int nw, const double* weight, const double* ubuf, DevScalars* sc, int* table, double* prob,
+                int* number, hipStream_t s) {
+  hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(64), 0, s, nw, weight, ubuf, sc, table, prob, number);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+int launch_apply_table(int nw, int nblk, void* const* wf, const long long* wsize, zd* logw, int* alive,
+                       const int* table, hipStream_t s) {
+  dim3 grid(8, nw);
+  hipLaunchKernelGGL(apply_table_kernel<T>, grid, dim3(256), 0, s, nw, reinterpret_cast<cplx<T>*>(wf[0]),
+                     nblk > 1 ? reinterpret_cast<cplx<T>*>(wf[1]) : nullptr, wsize[0], nblk > 1 ? wsize[1] : 0, logw,
+                     alive, table);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+template int launch_apply_table<double>(int, int, void* const*, const long long*, zd*, int*, const int*, hipStream_t);
+template int launch_apply_table<float>(int, int, void* const*, const long long*, zd*, int*, const int*, hipStream_t);
+
+int launch_energy(const EnergyArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(energy_kernel, dim3(a.nw), dim3(256), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(256), 0, s, a.nw, a.per_walker, a.sc);
+  HIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(site_reduce_kernel, dim3((a.L + 63) / 64), dim3(64), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_set_et_from_energy(DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(set_et_kernel, dim3(1), dim3(1), 0, s, sc);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int /*unused*/,
+                         double mult0, double mult1, hipStream_t s) {
+  hipLaunchKernelGGL(logw_add_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, nw, logw, alive, add0, add1, mult0,
+                     mult1);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}

@@ -1,0 +1,95 @@
+// Walker-level kernels of the constraint-path (CP) step and the comb.
+#pragma once
+#include "common.hpp"
+
+// decompType of NiupNidn (NiupNidn.cpp:201-226)
+enum { DECOMP_DENSITY_CHARGE = 0, DECOMP_DENSITY_SPIN = 1, DECOMP_HOP_CHARGE = 2, DECOMP_HOP_SPIN = 3 };
+
+// Device-resident scalars of a run: the driver loop never waits on the host.
+struct DevScalars {
+  double ET;
+  unsigned long long key;      // uniform stream key
+  unsigned long long counter;  // next uniform index
+  int alive_count;
+  int pad;
+  zd den, HNum, KNum, VNum, RNum;
+  zd NupTot, NdnTot, SplusTot, SminusTot;
+  double wmin, wmax, wavg;     // popCheck of the last comb
+  double taylor_orders;        // phaseless: total Taylor terms applied in the last step
+};
+
+__device__ __forceinline__ double afqmc_uniform_dev(unsigned long long key, unsigned long long k) {
+  unsigned long long z = key + (k + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+unsigned long long afqmc_seed_mix_host(unsigned long long seed);
+
+// logO combination of per-block log-determinants
+enum { COMBINE_WRAP = 0, COMBINE_SUM = 1 };
+// what to do with logO
+enum { LOGO_STEP = 0, LOGO_WEIGHT = 1, LOGO_MEASURE = 2, LOGO_ONLY = 3 };
+
+struct LogOArgs {
+  int nw, nblk, combine, mode;
+  const zd* ldet[2];
+  const int* info[2];
+  const zd* logw;        // walker logw (for MEASURE: logw of the temp walker)
+  zd* logw_out;          // updated in STEP / WEIGHT mode
+  int* alive;
+  zd trial_logw;
+  zd* logO;              // out
+  double* weight;        // WEIGHT mode out
+  zd* den;               // MEASURE mode out: exp(logO)
+};
+int launch_logo(const LogOArgs& a, hipStream_t s);
+
+int launch_alive_scan(int nw, const int* alive, int* prefix, DevScalars* sc, hipStream_t s);
+
+struct HsSampleArgs {
+  int nw, L, decomp, force_type, nblk, dense;
+  const zd* gd[2]; const zd* go[2];  // green partial sums [nw][ntile][Mb]
+  int ntile[2], Mb[2];
+  const zd* gamma;                   // L
+  const zd* cdiag;                   // 4 x L: 00, 01, 10, 11
+  const double* const_force;         // L
+  double cap, dt, log_half_L, dtUSum;
+  zd onebody_logw;                   // expMinusDtK.logw
+  const int* alive; const int* prefix;
+  DevScalars* sc;
+  const double* ubuf;                // host-provided uniforms (device copy) or nullptr
+  zd* logw;
+  void* scale[2]; int diag_only;     // complex T, [nw][Mb] per block
+  void* d2x2;                        // complex T, [nw][4][L] when !diag_only
+  double* force_out; int* aux_out;   // optional records
+  int fp32;
+};
+int launch_hs_sample(const HsSampleArgs& a, hipStream_t s);
+int launch_advance_counter(DevScalars* sc, int per_walker, hipStream_t s);
+
+template <typename T>
+int launch_hs_apply_2x2(int nw, int L, int N, const void* in, void* out, const void* d2x2, hipStream_t s);
+
+int launch_comb(int nw, const double* weight, const double* ubuf, DevScalars* sc, int* table, double* prob,
+                int* number, hipStream_t s);
+template <typename T>
+int launch_apply_table(int nw, int nblk, void* const* wf, const long long* wsize, zd* logw, int* alive,
+                       const int* table, hipStream_t s);
+
+struct EnergyArgs {
+  int nw, L, nblk, dense;
+  const zd* gd[2]; const zd* go[2]; const zd* gk[2];
+  int ntile[2], Mb[2];
+  const double *U, *mu, *hx, *hy, *hz;
+  const int* alive;
+  const zd* den;
+  zd* per_walker;      // [nw][8]: den, H, K, V, R, Nup, Ndn, (unused)
+  zd* site_acc;        // [4][L] += per site (Nup, Ndn, Splus, Sminus) * den
+  DevScalars* sc;
+};
+int launch_energy(const EnergyArgs& a, hipStream_t s);
+int launch_set_et_from_energy(DevScalars* sc, hipStream_t s);
+int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int use_imag_sub,
+                         double mult0, double mult1, hipStream_t s);

@@ -1,0 +1,282 @@
+// Batched complex GEMM kernels (MFMA 16x16x4, f64 or f32) for gfx950.
+//
+// The complex product is done with four real MFMAs per 16x16x4 block:
+//   Cre += Are*Bre - Aim*Bim,  Cim += Are*Bim + Aim*Bre
+// (8 real flops per complex multiply-add, as counted in the roofline).  Tiles
+// are staged through LDS as separate re/im planes so that each MFMA operand
+// is one LDS read per lane; the next K tile is prefetched into registers while
+// the current one is multiplied.
+#include "gemm.hpp"
+
+namespace {
+
+template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
+__global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
+  constexpr int NT = WM * WN * 64;
+  constexpr int TM = BM / WM, TN = BN / WN;
+  constexpr int FM = TM / 16, FN = TN / 16;
+  constexpr int PAD = 1;
+  constexpr int A_ELEMS = BM * BK / NT;
+  constexpr int B_ELEMS = BK * BN / NT;
+  static_assert(A_ELEMS * NT == BM * BK && B_ELEMS * NT == BK * BN, "tile/thread mismatch");
+  typedef typename mfma_traits<T>::acc_t acc_t;
+  typedef cplx<T> ct;
+
+  __shared__ T As[2][BK][BM + PAD];
+  __shared__ T Bs[2][BK][BN + PAD];
+
+  const int b = blockIdx.z;
+  if (p.alive && !p.alive[b]) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+
+  const ct* __restrict__ A = reinterpret_cast<const ct*>(p.A) + (long long)b * p.strideA;
+  const ct* __restrict__ Bm = reinterpret_cast<const ct*>(p.B) + (long long)b * p.strideB;
+  const ct* __restrict__ S = SCALE ? reinterpret_cast<const ct*>(p.scale) + (long long)b * p.scale_strideB : nullptr;
+
+  acc_t acc_re[FM][FN], acc_im[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acc_re[i][j] = acc_t{0, 0, 0, 0};
+      acc_im[i][j] = acc_t{0, 0, 0, 0};
+    }
+
+  ct ra[A_ELEMS], rb[B_ELEMS];
+
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < A_ELEMS; ++e) {
+      int idx = tid + e * NT;
+      int m, k;
+      if (OPA == OP_N) { m = idx % BM; k = idx / BM; }
+      else { k = idx % BK; m = idx / BK; }
+      int gm = m0 + m, gk = k0 + k;
+      ct v = cmk<T>(0, 0);
+      if (gm < p.M && gk < p.K) {
+        v = (OPA == OP_N) ? A[gm + (long long)gk * p.lda] : A[gk + (long long)gm * p.lda];
+      }
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      int idx = tid + e * NT;
+      int k = idx % BK, n = idx / BK;
+      int gk = k0 + k, gn = n0 + n;
+      ct v = cmk<T>(0, 0);
+      if (gk < p.K && gn < p.N) {
+        v = Bm[gk + (long long)gn * p.ldb];
+        if (SCALE) {
+          ct s = S[(long long)(gn / p.scale_group) * p.scale_strideG + gk];
+          v = cmul(s, v);
+        }
+      }
+      rb[e] = v;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int e = 0; e < A_ELEMS; ++e) {
+      int idx = tid + e * NT;
+      int m, k;
+      if (OPA == OP_N) { m = idx % BM; k = idx / BM; }
+      else { k = idx % BK; m = idx / BK; }
+      As[0][k][m] = ra[e].re;
+      As[1][k][m] = (OPA == OP_C) ? -ra[e].im : ra[e].im;
+    }
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      int idx = tid + e * NT;
+      int k = idx % BK, n = idx / BK;
+      Bs[0][k][n] = rb[e].re;
+      Bs[1][k][n] = rb[e].im;
+    }
+  };
+
+  const int nk = (p.K + BK - 1) / BK;
+  load_tiles(0);
+  store_tiles();
+  __syncthreads();
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T are[FM], aim[FM], naim[FM], bre[FN], bim[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        int m = wm * TM + i * 16 + lr;
+        are[i] = As[0][kk + lk][m];
+        aim[i] = As[1][kk + lk][m];
+        naim[i] = -aim[i];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        int n = wn * TN + j * 16 + lr;
+        bre[j] = Bs[0][kk + lk][n];
+        bim[j] = Bs[1][kk + lk][n];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          acc_re[i][j] = mfma_traits<T>::mma(are[i], bre[j], acc_re[i][j]);
+          acc_re[i][j] = mfma_traits<T>::mma(naim[i], bim[j], acc_re[i][j]);
+          acc_im[i][j] = mfma_traits<T>::mma(are[i], bim[j], acc_im[i][j]);
+          acc_im[i][j] = mfma_traits<T>::mma(aim[i], bre[j], acc_im[i][j]);
+        }
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store_tiles();
+      __syncthreads();
+    }
+  }
+
+  if (EPI == EPI_STORE) {
+    ct* C = reinterpret_cast<ct*>(p.C) + (long long)b * p.strideC;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        int gn = n0 + wn * TN + j * 16 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int gm = m0 + wm * TM + i * 16 + mfma_traits<T>::row(lk, r);
+          if (gm < p.M && gn < p.N) {
+            double xr = (double)acc_re[i][j][r], xi = (double)acc_im[i][j][r];
+            double yr = p.alpha_re * xr - p.alpha_im * xi;
+            double yi = p.alpha_re * xi + p.alpha_im * xr;
+            long long off = gm + (long long)gn * p.ldc;
+            if (p.beta_one) {
+              ct o = C[off];
+              yr += (double)o.re;
+              yi += (double)o.im;
+            }
+            C[off] = cmk<T>((T)yr, (T)yi);
+          }
+        }
+      }
+  } else {
+    // green epilogue: row dots of X with the partner matrices, reduced over
+    // this tile's columns, written as partial sums per column tile.
+    const ct* P0 = reinterpret_cast<const ct*>(p.P0);
+    const ct* P2 = reinterpret_cast<const ct*>(p.P2);
+    __shared__ zd red[3][WN][BM];
+    zd d0[FM][4], d1[FM][4], d2[FM][4];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        d0[i][r] = cmk<double>(0, 0);
+        d1[i][r] = cmk<double>(0, 0);
+        d2[i][r] = cmk<double>(0, 0);
+      }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        int gn = n0 + wn * TN + j * 16 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int gm = m0 + wm * TM + i * 16 + mfma_traits<T>::row(lk, r);
+          if (gm < p.M && gn < p.N) {
+            zd x = cmk<double>((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
+            zd q0 = to_zd(P0[gm + (long long)gn * p.ldp]);
+            d0[i][r] = cadd(d0[i][r], cmul(x, cconj(q0)));
+            if (p.partner_half) {
+              int gp = gm < p.partner_half ? gm + p.partner_half : gm - p.partner_half;
+              zd q1 = to_zd(P0[gp + (long long)gn * p.ldp]);
+              d1[i][r] = cadd(d1[i][r], cmul(x, cconj(q1)));
+            }
+            if (P2) {
+              zd q2 = to_zd(P2[gm + (long long)gn * p.ldp]);
+              d2[i][r] = cadd(d2[i][r], cmul(x, q2));
+            }
+          }
+        }
+      }
+    // reduce over the 16 lanes that share a row (lane % 16 = column)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+          d0[i][r].re += __shfl_xor(d0[i][r].re, o, 64);
+          d0[i][r].im += __shfl_xor(d0[i][r].im, o, 64);
+          d1[i][r].re += __shfl_xor(d1[i][r].re, o, 64);
+          d1[i][r].im += __shfl_xor(d1[i][r].im, o, 64);
+          d2[i][r].re += __shfl_xor(d2[i][r].re, o, 64);
+          d2[i][r].im += __shfl_xor(d2[i][r].im, o, 64);
+        }
+        if (lr == 0) {
+          int m = wm * TM + i * 16 + mfma_traits<T>::row(lk, r);
+          red[0][wn][m] = d0[i][r];
+          red[1][wn][m] = d1[i][r];
+          red[2][wn][m] = d2[i][r];
+        }
+      }
+    __syncthreads();
+    const long long ntn = gridDim.y;
+    for (int m = tid; m < BM; m += NT) {
+      int gm = m0 + m;
+      if (gm >= p.M) continue;
+      zd s0 = cmk<double>(0, 0), s1 = s0, s2 = s0;
+      for (int w = 0; w < WN; ++w) {
+        s0 = cadd(s0, red[0][w][m]);
+        s1 = cadd(s1, red[1][w][m]);
+        s2 = cadd(s2, red[2][w][m]);
+      }
+      long long off = ((long long)b * ntn + blockIdx.y) * p.M + gm;
+      p.out0[off] = s0;
+      if (p.out1) p.out1[off] = s1;
+      if (p.out2) p.out2[off] = s2;
+    }
+  }
+}
+
+template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
+int run(const GemmArgs& a, hipStream_t s) {
+  dim3 grid((a.M + BM - 1) / BM, (a.N + BN - 1) / BN, a.batch);
+  if (grid.y > 65535 || grid.z > 65535) {
+    afqmc_set_error("cgemm grid too large (%u, %u, %u)", grid.x, grid.y, grid.z);
+    return -1;
+  }
+  hipLaunchKernelGGL((cgemm_kernel<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>), grid, dim3(WM * WN * 64), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T, int OPA, int EPI, bool SCALE>
+int dispatch_tile(const GemmArgs& a, hipStream_t s) {
+  // 64x64 output tiles, 4 waves (2x2), K tile 16: 32 accumulator values per lane
+  return run<T, OPA, 64, 64, 16, 2, 2, EPI, SCALE>(a, s);
+}
+
+}  // namespace
+
+template <typename T>
+int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s) {
+  if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return 0;
+  REQUIRE(a.K >= 0, "cgemm: negative K");
+  const bool sc = a.scale != nullptr;
+  REQUIRE(!sc || a.scale_group > 0, "cgemm: scale_group must be positive");
+  REQUIRE(epi == EPI_STORE || (a.out0 && a.P0), "cgemm: green epilogue needs P0/out0");
+  if (epi == EPI_STORE) {
+    if (opa == OP_N) return sc ? dispatch_tile<T, OP_N, EPI_STORE, true>(a, s) : dispatch_tile<T, OP_N, EPI_STORE, false>(a, s);
+    if (opa == OP_T) return dispatch_tile<T, OP_T, EPI_STORE, false>(a, s);
+    if (opa == OP_C) return dispatch_tile<T, OP_C, EPI_STORE, false>(a, s);
+  } else {
+    if (opa == OP_N) return dispatch_tile<T, OP_N, EPI_GREEN, false>(a, s);
+  }
+  afqmc_set_error("cgemm: unsupported op/epilogue combination (%d, %d)", opa, epi);
+  return -1;
+}
+
+template int launch_cgemm<double>(int, int, const GemmArgs&, hipStream_t);
+template int launch_cgemm<float>(int, int, const GemmArgs&, hipStream_t);

@@ -1,0 +1,36 @@
+// Batched complex GEMM on MFMA for gfx950.
+//
+//   C_b = alpha * op(A_b) * diag(s_{b,n}) * B_b          (EPI_STORE)
+//   green row-dots of X_b = op(A_b) * B_b                (EPI_GREEN)
+//
+// op(A) in {N, T, C}; any of A, B may be shared across the batch (stride 0).
+// "Giant" mode: one batch whose N dimension concatenates all walkers' columns
+// (walkers are stored back to back with the same leading dimension), with the
+// per-walker row scaling selected by column group n / scale_group.
+#pragma once
+#include "common.hpp"
+
+enum { OP_N = 0, OP_T = 1, OP_C = 2 };
+enum { EPI_STORE = 0, EPI_GREEN = 1 };
+
+struct GemmArgs {
+  int M, N, K, batch;
+  const void* A; long long lda, strideA;
+  const void* B; long long ldb, strideB;
+  void* C; long long ldc, strideC;
+  double alpha_re, alpha_im;
+  int beta_one;            // C += result instead of C = result
+  // optional row scaling of B (complex T): s = scale + b*scale_strideB + (n/scale_group)*scale_strideG
+  const void* scale; long long scale_strideB, scale_strideG; int scale_group;
+  // green epilogue partners (complex T, M x N column-major, leading dim ldp, shared across batch)
+  const void* P0;          // trial wf: d0 = sum_j X(i,j) conj(P0(i,j)),  d1 = sum_j X(i,j) conj(P0(i^,j))
+  const void* P2;          // (phiT^H K)^T: d2 = sum_j X(i,j) P2(i,j)
+  long long ldp;
+  int partner_half;        // i^ = i +- partner_half for the off-diagonal pairs (0 = none)
+  zd* out0; zd* out1; zd* out2;  // [batch][ntileN][M] partial sums
+  // alive mask: batches with alive[b]==0 are skipped (batched mode only); nullptr = all
+  const int* alive;
+};
+
+template <typename T>
+int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s);

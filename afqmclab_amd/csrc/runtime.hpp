@@ -1,0 +1,84 @@
+// Device context, owning device buffers and small elementwise helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "../../include/afqmc_mi355x.h"
+#include "common.hpp"
+#include "cp_kernels.hpp"
+
+struct afqmc_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+// Selects the context's device for the lifetime of a C entry point.
+struct ScopedDevice {
+  int prev = 0;
+  explicit ScopedDevice(afqmc_ctx ctx) {
+    hipGetDevice(&prev);
+    if (ctx && ctx->device != prev) hipSetDevice(ctx->device);
+  }
+  explicit ScopedDevice(int device) {
+    hipGetDevice(&prev);
+    if (device != prev) hipSetDevice(device);
+  }
+  ~ScopedDevice() {
+    int cur = 0;
+    hipGetDevice(&cur);
+    if (cur != prev) hipSetDevice(prev);
+  }
+};
+
+// Owning device allocation (hipMalloc / hipFree), movable.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) { return alloc_bytes(count * sizeof(T)); }
+  int alloc_bytes(size_t bytes) {
+    release();
+    if (bytes == 0) bytes = 16;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+      afqmc_set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+      return -3;
+    }
+    p = reinterpret_cast<T*>(q);
+    n = bytes / sizeof(T);
+    return 0;
+  }
+};
+template <typename T>
+void swap(DevBuf<T>& a, DevBuf<T>& b) noexcept {
+  std::swap(a.p, b.p);
+  std::swap(a.n, b.n);
+}
+
+int launch_convert_f32_f64(const void* in, void* out, long long ncomplex, hipStream_t s);
+int launch_convert_f64_f32(const void* in, void* out, long long ncomplex, hipStream_t s);
+// logw_out[w] = logw[w] + c
+int launch_logw_shift(int nw, const zd* logw, zd* logw_out, zd c, hipStream_t s);
+// logw[w] += c for alive walkers
+int launch_logw_const_add(int nw, zd* logw, const int* alive, zd c, hipStream_t s);
+// logw[w] += v[w] for alive walkers
+int launch_logw_vec_add(int nw, zd* logw, const int* alive, const zd* v, hipStream_t s);
+// dst[w] = src[w] (per-walker blocks of n elements of size esz) for alive walkers
+int launch_select_alive(int nw, long long n, size_t esz, const int* alive, const void* src, void* dst, hipStream_t s);
+int launch_reset_measure(DevScalars* sc, zd* site_acc, int L, hipStream_t s);

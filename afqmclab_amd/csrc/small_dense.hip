@@ -1,0 +1,350 @@
+// Per-walker N x N dense kernels, register resident, one workgroup per matrix.
+//
+// The N x N complex FP64 matrix (N <= 160; 112 x 112 is 196 KB, more than a
+// CU's LDS) is distributed 2D-cyclically over a TG x TG thread grid: thread
+// (tr, tc) owns rows tr + TG*a and columns tc + TG*b.  Only one column and one
+// row per elimination step travel through LDS (double buffered), pivots are
+// found with wavefront shuffles (every wave reduces redundantly, so no extra
+// barrier is needed for the pivot index).
+//
+// gj_inverse_logdet: Gauss-Jordan exchange sweep with partial pivoting (the
+//   pivot of column k is the largest |re|+|im| among rows not yet used, first
+//   index on ties, as izamax in zgetrf).  The pivots equal LU's u_kk, so
+//   logdet = sum log|u_kk| + log(prod u_kk/|u_kk| * sign(perm)) reproduces
+//   logDeterminant (tensor_2d_common_fun.cpp:75-105).  Writes O^{-1}.
+// chol_inverse: S = R^H R (upper R, positive real diagonal), then R^{-1} by
+//   the same sweep without pivoting; returns sum log R_kk (CholeskyQR step of
+//   the walker restabilisation).
+#include "small_dense.hpp"
+
+namespace {
+
+constexpr int NMAX = 160;
+
+__device__ __forceinline__ double abs1(zd a) { return fabs(a.re) + fabs(a.im); }
+
+template <typename T, int TG, int E>
+__global__ __launch_bounds__(TG * TG) void gj_kernel(SmallDenseArgs p) {
+  const int bidx = blockIdx.x;
+  if (p.alive && !p.alive[bidx / p.alive_div]) return;
+  const int N = p.N;
+  const int tid = threadIdx.x;
+  const int tr = tid % TG, tc = tid / TG;
+  const int lane = tid & 63;
+  typedef cplx<T> ct;
+  const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
+  ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
+
+  __shared__ zd colbuf[2][NMAX];
+  __shared__ zd rowbuf[2][NMAX];
+  __shared__ int perm[NMAX];
+  __shared__ int qinv[NMAX];
+  __shared__ unsigned char seen[NMAX];
+
+  zd x[E][E];
+#pragma unroll
+  for (int a = 0; a < E; ++a)
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int i = tr + TG * a, j = tc + TG * c;
+      x[a][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+    }
+  // rows handled by this lane in the pivot search: lane, lane+64, lane+128
+  unsigned used = 0u;
+  double lognorm = 0.0;
+  zd phase = cmk<double>(1.0, 0.0);
+  int singular = 0;
+
+  for (int k = 0; k < N; ++k) {
+    const int buf = k & 1;
+    if (tc == k % TG) {
+      const int kc = k / TG;
+#pragma unroll
+      for (int a = 0; a < E; ++a) {
+        int i = tr + TG * a;
+#pragma unroll
+        for (int c = 0; c < E; ++c)
+          if (c == kc && i < N) colbuf[buf][i] = x[a][c];
+      }
+    }
+    __syncthreads();
+    // pivot search (redundant in every wave)
+    double bv = -1.0;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int s = 0; s < (NMAX + 63) / 64; ++s) {
+      int i = lane + 64 * s;
+      if (i < N && !((used >> s) & 1u)) {
+        double v = abs1(colbuf[buf][i]);
+        if (!(v >= 0.0)) v = 0.0;  // NaN input: still pick a valid unused row (reported singular)
+        if (v > bv) { bv = v; bi = i; }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      double ov = __shfl_xor(bv, o, 64);
+      int oi = __shfl_xor(bi, o, 64);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    // at step k there are N-k >= 1 unused rows and every one of them scores >= 0 > -1, so
+    // bi is a valid row; the clamp keeps every later index in range regardless
+    const int pr = (bi >= 0 && bi < N) ? bi : 0;
+    if ((pr & 63) == lane) used |= 1u << (pr >> 6);
+    const zd piv = colbuf[buf][pr];
+    zd inv = cmk<double>(0, 0);
+    if (bv > 0.0) {
+      inv = cinv(piv);
+    } else {
+      singular = 1;
+    }
+    if (tid == 0) {
+      perm[k] = pr;
+      if (!singular) {
+        double ap = cabs(piv);
+        lognorm += log(ap);
+        phase = cmul(phase, cmk<double>(piv.re / ap, piv.im / ap));
+      }
+    }
+    if (tr == pr % TG) {
+      const int ar = pr / TG;
+#pragma unroll
+      for (int a = 0; a < E; ++a) {
+        if (a != ar) continue;
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+          int j = tc + TG * c;
+          if (j < N) rowbuf[buf][j] = (j == k) ? cmk<double>(0, 0) : cmul(x[a][c], inv);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < E; ++a) {
+      int i = tr + TG * a;
+      if (i >= N) continue;
+      zd ci = colbuf[buf][i];
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        int j = tc + TG * c;
+        if (j >= N) continue;
+        zd r = rowbuf[buf][j];
+        if (i == pr) {
+          x[a][c] = (j == k) ? inv : cmk<double>(-r.re, -r.im);
+        } else if (j == k) {
+          x[a][c] = cmul(ci, inv);
+        } else {
+          x[a][c] = csub(x[a][c], cmul(ci, r));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < N) qinv[perm[tid]] = tid;
+  if (tid == 0) {
+    // sign of the permutation k -> perm[k]
+    int visited_parity = 0;
+    for (int i = 0; i < N; ++i) seen[i] = 0;
+    for (int i = 0; i < N; ++i) {
+      if (seen[i]) continue;
+      int len = 0, j = i;
+      // perm is a permutation of 0..N-1 (each row is used once), so a cycle has at most N
+      // elements; the length bound keeps the walk finite even if that were violated
+      while (len < N && !seen[j]) { seen[j] = 1; j = perm[j]; ++len; }
+      visited_parity ^= (len - 1) & 1;
+    }
+    zd ph = visited_parity ? cmk<double>(-phase.re, -phase.im) : phase;
+    zd ld;
+    if (singular) {
+      ld = cmk<double>(-1e300, 0.0);
+    } else {
+      ld = cmk<double>(lognorm + 0.0, atan2(ph.im, ph.re));
+      // log(phase) of a unit complex: log|ph| is rounding-level; keep it as the reference does
+      ld.re += log(hypot(ph.re, ph.im));
+    }
+    p.logdet[bidx] = ld;
+    if (p.info) p.info[bidx] = singular;
+  }
+  if (!p.write_inverse) return;
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < E; ++a) {
+    int i = tr + TG * a;
+    if (i >= N) continue;
+    int row = qinv[i];
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int j = tc + TG * c;
+      if (j >= N) continue;
+      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(x[a][c]);
+    }
+  }
+}
+
+template <typename T, int TG, int E>
+__global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
+  const int bidx = blockIdx.x;
+  if (p.alive && !p.alive[bidx / p.alive_div]) return;
+  const int N = p.N;
+  const int tid = threadIdx.x;
+  const int tr = tid % TG, tc = tid / TG;
+  typedef cplx<T> ct;
+  const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
+  ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
+
+  __shared__ zd colbuf[2][NMAX];
+  __shared__ zd rowbuf[2][NMAX];
+
+  zd x[E][E];
+#pragma unroll
+  for (int a = 0; a < E; ++a)
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int i = tr + TG * a, j = tc + TG * c;
+      x[a][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+    }
+  double logsum = 0.0, logsub = 0.0;
+  int bad = 0;
+  // Cholesky, upper factor: one barrier per step (double-buffered row)
+  for (int k = 0; k < N; ++k) {
+    const int buf = k & 1;
+    if (tr == k % TG) {
+      const int ar = k / TG;
+#pragma unroll
+      for (int a = 0; a < E; ++a) {
+        if (a != ar) continue;
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+          int j = tc + TG * c;
+          if (j < N) rowbuf[buf][j] = x[a][c];
+        }
+      }
+    }
+    __syncthreads();
+    double skk = rowbuf[buf][k].re;
+    if (!(skk > 0.0)) { bad = 1; skk = 1.0; }
+    const double d = sqrt(skk), id = 1.0 / d;
+    if (tid == 0) {
+      double l = log(d);
+      logsum += l;
+      if (k < p.nsub) logsub += l;
+    }
+#pragma unroll
+    for (int a = 0; a < E; ++a) {
+      int i = tr + TG * a;
+      if (i >= N || i < k) continue;
+      zd rki = cscale(rowbuf[buf][i], id);  // R[k,i]
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        int j = tc + TG * c;
+        if (j >= N || j < k) continue;
+        zd rkj = cscale(rowbuf[buf][j], id);
+        if (i == k) x[a][c] = rkj;
+        else x[a][c] = csub(x[a][c], cmul(cconj(rki), rkj));
+      }
+    }
+  }
+  // zero the strictly lower part
+#pragma unroll
+  for (int a = 0; a < E; ++a)
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int i = tr + TG * a, j = tc + TG * c;
+      if (j < i) x[a][c] = cmk<double>(0, 0);
+    }
+  // R^{-1} by the exchange sweep with pivot (k, k)
+  for (int k = 0; k < N; ++k) {
+    const int buf = k & 1;
+    __syncthreads();
+    if (tc == k % TG) {
+      const int kc = k / TG;
+#pragma unroll
+      for (int a = 0; a < E; ++a) {
+        int i = tr + TG * a;
+#pragma unroll
+        for (int c = 0; c < E; ++c)
+          if (c == kc && i < N) colbuf[buf][i] = x[a][c];
+      }
+    }
+    if (tr == k % TG) {
+      const int ar = k / TG;
+#pragma unroll
+      for (int a = 0; a < E; ++a) {
+        if (a != ar) continue;
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+          int j = tc + TG * c;
+          if (j < N) rowbuf[buf][j] = x[a][c];
+        }
+      }
+    }
+    __syncthreads();
+    const zd inv = cinv(rowbuf[buf][k]);
+#pragma unroll
+    for (int a = 0; a < E; ++a) {
+      int i = tr + TG * a;
+      if (i >= N) continue;
+      zd ci = colbuf[buf][i];
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        int j = tc + TG * c;
+        if (j >= N) continue;
+        zd r = cmul(rowbuf[buf][j], inv);
+        if (i == k) {
+          x[a][c] = (j == k) ? inv : cmk<double>(-r.re, -r.im);
+        } else if (j == k) {
+          x[a][c] = cmul(ci, inv);
+        } else {
+          x[a][c] = csub(x[a][c], cmul(ci, r));
+        }
+      }
+    }
+  }
+  if (tid == 0) {
+    p.logdet[bidx] = cmk<double>(logsum, logsub);
+    if (p.info) p.info[bidx] = bad;
+  }
+#pragma unroll
+  for (int a = 0; a < E; ++a) {
+    int i = tr + TG * a;
+    if (i >= N) continue;
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int j = tc + TG * c;
+      if (j >= N) continue;
+      out[i + (long long)j * p.ld_out] = from_zd<T>(x[a][c]);
+    }
+  }
+}
+
+template <typename T, int TG, int E>
+int run(int kind, const SmallDenseArgs& a, hipStream_t s) {
+  dim3 grid(a.batch), block(TG * TG);
+  if (kind == 0) hipLaunchKernelGGL((gj_kernel<T, TG, E>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((chol_kernel<T, TG, E>), grid, block, 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
+  REQUIRE(a.N >= 1 && a.N <= NMAX, "small dense kernels support 1 <= N <= %d (got %d)", NMAX, a.N);
+  REQUIRE(a.alive_div >= 1, "alive_div must be >= 1");
+  if (a.batch <= 0) return 0;
+  if (a.N <= 16) return run<T, 8, 2>(kind, a, s);
+  if (a.N <= 32) return run<T, 16, 2>(kind, a, s);
+  if (a.N <= 64) return run<T, 16, 4>(kind, a, s);
+  if (a.N <= 128) return run<T, 32, 4>(kind, a, s);
+  return run<T, 32, 5>(kind, a, s);
+}
+
+}  // namespace
+
+template <typename T>
+int launch_gj_inverse(const SmallDenseArgs& a, hipStream_t s) { return dispatch<T>(0, a, s); }
+template <typename T>
+int launch_chol_inverse(const SmallDenseArgs& a, hipStream_t s) { return dispatch<T>(1, a, s); }
+
+template int launch_gj_inverse<double>(const SmallDenseArgs&, hipStream_t);
+template int launch_gj_inverse<float>(const SmallDenseArgs&, hipStream_t);
+template int launch_chol_inverse<double>(const SmallDenseArgs&, hipStream_t);
+template int launch_chol_inverse<float>(const SmallDenseArgs&, hipStream_t);

@@ -1,0 +1,17 @@
+#pragma once
+#include "common.hpp"
+
+struct SmallDenseArgs {
+  int N, batch;
+  const void* in; long long ld_in, stride_in;   // complex T, N x N column-major per matrix
+  void* out; long long ld_out, stride_out;      // complex T (may alias in)
+  zd* logdet;          // gj: log det (principal branch); chol: (sum log R_kk, sum_{k<nsub} log R_kk)
+  int* info;           // gj: 1 if singular; chol: 1 if not positive definite
+  int write_inverse;   // gj only
+  int nsub;            // chol only
+  const int* alive;    // optional: skip matrix b if alive[b / alive_div] == 0
+  int alive_div;
+};
+
+template <typename T> int launch_gj_inverse(const SmallDenseArgs& a, hipStream_t s);
+template <typename T> int launch_chol_inverse(const SmallDenseArgs& a, hipStream_t s);

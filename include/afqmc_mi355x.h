@@ -1,0 +1,150 @@
+/*
+ * C-ABI of the MI355X walker-propagation library (libafqmc_mi355x.so).
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved (re, im)
+ * doubles and matrices column-major, the layout of the reference's
+ * TensorHao<std::complex<double>,2> (common/tensorHao/include/tensor_core.h:85-106),
+ * so a reference SD's wf.data() can be passed as is.  Every function returns
+ * 0 on success and a negative code on error; afqmc_last_error() then holds the
+ * message (the reference prints and calls exit(1); a library must not exit).
+ * Calls are host-synchronous unless stated otherwise.  Walker state lives in
+ * HBM between calls.
+ *
+ * Each entry point names the reference function it replaces, batched over
+ * all walkers of a set (the reference loops over walkers one at a time).
+ */
+#ifndef AFQMC_MI355X_H
+#define AFQMC_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afqmc_ctx_s* afqmc_ctx;
+typedef struct afqmc_cp_s* afqmc_cp;
+typedef struct afqmc_walkers_s* afqmc_walkers;
+typedef struct afqmc_ph_s* afqmc_ph;
+
+/* ------------------------------------------------------------------ context */
+const char* afqmc_last_error(void);
+int afqmc_version(void);
+/* device context and stream (all work of one context is ordered on one HIP stream) */
+int afqmc_ctx_create(int device, afqmc_ctx* out);
+int afqmc_ctx_destroy(afqmc_ctx ctx);
+int afqmc_ctx_synchronize(afqmc_ctx ctx);
+
+/* ------------------------------------------------------------------ CP (Hubbard) */
+/* Model + method description.  Mirrors HubbardSOC (HubbardSOC.cpp:39-59), the
+ * three Hop propagators built by returnExpMinusAlphaK (HubbardSOC.cpp:109-118),
+ * NiupNidn's parameters (NiupNidn.cpp:12-26) and the CP method keys used on the
+ * hot path (afqmcConstraintPathMethod.cpp:19-46). */
+typedef struct {
+  int L;                 /* lattice sites; walkers are 2L x N */
+  int N;                 /* particles (HubbardSOC::N) */
+  int Nup;               /* spin_blocks: first Nup columns are spin up */
+  int spin_blocks;       /* 1: walkers and trial are block diagonal (rows 0..L-1 x cols 0..Nup-1 and
+                            rows L..2L-1 x cols Nup..N-1), K and the HS field do not mix spins;
+                            stored and multiplied as two blocks.  0: dense 2L x N. */
+  int precision;         /* 0: complex FP64 throughout; 1: complex FP32 walkers and GEMMs, FP64 LU,
+                            log-determinants, QR, weights and energies */
+  int decomp_type;       /* NiupNidn decompType: 0 densityCharge, 1 densitySpin, 2 hopCharge, 3 hopSpin */
+  int force_type;        /* 0 dynamicForce, 1 constForce */
+  double dt;
+  double force_cap;
+  const double* K;                    /* 2L x 2L complex */
+  const double* exp_minus_dt_k;       /* 2L x 2L complex: exp(-dt K) */
+  const double* exp_minus_half_dt_k;  /* exp(-dt/2 K) */
+  const double* exp_half_dt_k;        /* exp(+dt/2 K) */
+  double hop_logw[6];                 /* logw (re, im) of the three Hop operators */
+  const double* U;                    /* L */
+  const double* mu;                   /* L */
+  const double* hx;                   /* L */
+  const double* hy;                   /* L */
+  const double* hz;                   /* L */
+  const double* phiT;                 /* 2L x N complex trial wave function */
+  double phiT_logw[2];
+  const double* const_force;          /* L, used when force_type == 1 (NiupNidn::readForce) */
+  int max_walkers;                    /* workspace is sized for this many walkers */
+} afqmc_cp_desc;
+
+int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* desc, afqmc_cp* out);
+int afqmc_cp_destroy(afqmc_cp cp);
+/* gamma and constDiag of the NiupNidn built from desc (NiupNidn.cpp:201-250): 5 x L complex */
+int afqmc_cp_get_hs_constants(afqmc_cp cp, double* gamma_cdiag);
+
+/* walker sets: std::vector<SD> of the driver (afqmcConstraintPath.h:27-28) */
+int afqmc_cp_walkers_create(afqmc_cp cp, int nw, afqmc_walkers* out);
+int afqmc_walkers_destroy(afqmc_walkers w);
+int afqmc_walkers_count(afqmc_walkers w);
+/* wf: nw x (2L x N) complex, logw: nw complex, alive: nw ints (NULL = all alive) */
+int afqmc_walkers_upload(afqmc_walkers w, const double* wf, const double* logw, const int* alive);
+int afqmc_walkers_download(afqmc_walkers w, double* wf, double* logw, int* alive);
+
+/* random numbers: a counter-based uniform stream on the device, or host-provided
+ * uniforms in the reference's consumption order (uniformHao, random_hao.cpp:73-76) */
+int afqmc_cp_set_seed(afqmc_cp cp, uint64_t seed, uint64_t counter);
+int afqmc_cp_get_counter(afqmc_cp cp, uint64_t* counter);
+int afqmc_cp_set_uniforms(afqmc_cp cp, const double* u, int64_t n); /* n = 0: back to the device stream */
+/* consume n uniforms without using them (e.g. the aux sample drawn by estimateMemory,
+ * afqmcConstraintPath.cpp:66) */
+int afqmc_cp_skip_uniforms(afqmc_cp cp, uint64_t n);
+int afqmc_cp_set_ET(afqmc_cp cp, double ET);
+int afqmc_cp_get_ET(afqmc_cp cp, double* ET);
+
+/* SDSDOperation::returnLogOverlap (SDSDOperation.cpp:65-72) for every walker: logO nw complex */
+int afqmc_cp_log_overlap(afqmc_cp cp, afqmc_walkers w, double* logO);
+/* SDSDOperation::returnGreenDiagonal / returnGreenOffDiagonal (SDSDOperation.cpp:91-141):
+ * gdiag, goff: nw x 2L complex (goff may be NULL; zero in spin-block mode) */
+int afqmc_cp_green_diagonal(afqmc_cp cp, afqmc_walkers w, double* gdiag, double* goff);
+/* HubbardSOCMeasureFixedSDSD::getForce (HubbardSOCMeasureFixedSDSD.cpp:73-116): force nw x L */
+int afqmc_cp_force(afqmc_cp cp, afqmc_walkers w, double* force);
+/* HopSDOperation::applyToRight (hopSDOperation.cpp:14-19) with
+ * which = 0: exp(-dt K), 1: exp(-dt/2 K), 2: exp(+dt/2 K); alive walkers only */
+int afqmc_cp_onebody(afqmc_cp cp, afqmc_walkers w, int which);
+/* NiupNidnSampleSDOperation::applyToRight (NiupNidnSDOperation.cpp:15-41) with one sample per
+ * walker: diag4 = nw x 4 x L complex (diag00, diag01, diag10, diag11), sample_logw nw complex.
+ * Dense (spin_blocks = 0) sets only. */
+int afqmc_cp_hs_apply(afqmc_cp cp, afqmc_walkers w, const double* diag4, const double* sample_logw);
+/* SD::stabilize (SD.cpp:42-45) for every alive walker (CholeskyQR2, R with positive diagonal) */
+int afqmc_cp_stabilize(afqmc_cp cp, afqmc_walkers w);
+/* popConfiguration(1, weight) (populationControlConfiguration.cpp:24-101), one uniform per slot;
+ * stats = popCheck min, max, mean (may be NULL) */
+int afqmc_comb(afqmc_ctx ctx, int nw, const double* weight, const double* uniforms, int* table, double* stats);
+/* populationControl, serial branch (populationControl.h:110-116): walker[i] = walker[table[i]] */
+int afqmc_cp_apply_table(afqmc_cp cp, afqmc_walkers w, const int* table);
+
+/* ---- driver level: the loops of afqmcConstraintPathProjection.cpp / ...Measure.cpp ----
+ * These stay on the device; they do not synchronise unless an output pointer is given. */
+/* projectExpMinusDtKExpMinusDtV (:37-74): force (nw x L) and aux (nw x L) records may be NULL */
+int afqmc_cp_step(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out);
+/* popControl (:103-156): weights (nw), table (nw), stats (3) may be NULL */
+int afqmc_cp_pop_control(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats);
+/* addMixedMeasurement (afqmcConstraintPathMeasure.cpp:9-27) */
+int afqmc_cp_measure(afqmc_cp cp, afqmc_walkers w);
+/* accumulated den, HNum, KNum, VNum, RNum, NupTotNum, NdnTotNum: 7 complex */
+int afqmc_cp_energy(afqmc_cp cp, double* out);
+/* NupNum, NdnNum, SplusNum, SminusNum: 4 x L complex */
+int afqmc_cp_site_measure(afqmc_cp cp, double* out);
+int afqmc_cp_reset_measure(afqmc_cp cp);
+/* adjustETAndResetMeasurement (afqmcConstraintPathMeasure.cpp:35-45): ET = Re(HNum/den) on device, then reset */
+int afqmc_cp_adjust_ET(afqmc_cp cp);
+
+/* ------------------------------------------------------------------ testing hooks
+ * One kernel family on host data, for unit tests of the device building blocks.
+ * No reference counterpart: tensor_2d_bl_cpu.cpp's gmm / LUconstruct+inverse /
+ * QR are what they are checked against. */
+/* C_b = op(A_b) diag(scale) B_b; A M x K (op N) or K x M (op T/C), strideA = 0 shares A */
+int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K, int batch, const double* A,
+                     long long strideA, const double* B, const double* scale, int scale_group, double* C);
+/* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting) */
+int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,
+                          double* logdet, int* info);
+/* batched Cholesky S = R^H R, returns R^{-1} and (sum log R_kk, same) */
+int afqmc_test_chol_inverse(afqmc_ctx ctx, int N, int batch, const double* S, double* Rinv, double* logsum,
+                            int* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

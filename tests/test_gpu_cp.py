@@ -1,0 +1,241 @@
+"""Parity of the CP walker path on the GPU against the CPU oracle, through the C-ABI.
+
+FP64 bar: log-overlaps, weights, forces, Green's functions and energies to
+relative 1e-10; HS fields and comb tables exact.  Walker matrices are compared
+through invariants where the QR differs by column signs (CholeskyQR gives
+R_kk > 0, LAPACK's Householder may not): Q_dev^H Q_ref must be diagonal with
+unit-modulus entries."""
+import cmath
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from afqmclab_amd.cp import AfqmcConstraintPath, CPMethod as DevMethod, CPPropagator  # noqa: E402
+from afqmclab_amd.model import HubbardModel, free_electron_trial, spin_block_trial  # noqa: E402
+from oracle import comb as ocomb  # noqa: E402
+from oracle.blocks import SD, SDSDOperation, niupnidn_apply_right, NiupNidnSample  # noqa: E402
+from oracle.drivers import AfqmcConstraintPath as OracleCP, CPMethod as OracleMethod  # noqa: E402
+from oracle.models import HubbardSOC, HubbardSOCMeasureFixedSDSD  # noqa: E402
+from oracle.rng import ListStream, UniformStream  # noqa: E402
+from tests.gpu_util import crand, ctx  # noqa: E402
+
+RTOL = 1e-10
+
+
+def make_case(mode, nx=4, ny=4, nup=5, ndn=5, U=4.0, t2=0.0, decomp="densitySpin", precision=0, nw=12, seed=1):
+    model = HubbardModel.square(nx, ny, nup + ndn, U, 1.0, t2)
+    omodel = HubbardSOC(model.L, model.N, model.K, U=model.U)
+    if mode == "dense":
+        phi = free_electron_trial(model)
+        kw = dict(spin_blocks=False)
+    else:
+        phi = spin_block_trial(model, nup, ndn)
+        kw = dict(nup=nup, spin_blocks=True)
+    method = DevMethod(dt=0.05, decompType=decomp, walkerSizePerThread=nw)
+    prop = CPPropagator(ctx(), model, method, phi, precision=precision, max_walkers=nw, **kw)
+    rng = np.random.default_rng(seed)
+    L = model.L
+    wfs = []
+    for i in range(nw):
+        if mode == "dense":
+            w = phi + 0.3 * crand(rng, 2 * L, model.N)
+        else:
+            w = np.zeros_like(phi)
+            w[:L, :nup] = phi[:L, :nup] + 0.3 * crand(rng, L, nup)
+            w[L:, nup:] = phi[L:, nup:] + 0.3 * crand(rng, L, ndn)
+        wfs.append(w)
+    logw = 0.1 * crand(rng, nw)
+    ws = prop.walkers(nw)
+    ws.upload(wfs, logw)
+    return model, omodel, phi, prop, ws, wfs, logw, method
+
+
+def close_log(a, b, rtol=RTOL):
+    """log-overlaps: real parts relative, phases via exp(i Im)."""
+    assert abs(a.real - b.real) <= rtol * max(1.0, abs(b.real)), (a, b)
+    assert abs(cmath.exp(1j * (a.imag - b.imag)) - 1.0) <= rtol * 10, (a, b)
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_log_overlap_green_force(mode):
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode)
+    logO = prop.log_overlap(ws)
+    gd, go = prop.green_diagonal(ws)
+    f = prop.force(ws)
+    left = SD(wf=phi)
+    hs = om.returnExpMinusAlphaV(method.dt, method.decompType)
+    meas = HubbardSOCMeasureFixedSDSD(om, left)
+    for i in range(ws.nw):
+        op = SDSDOperation(left, SD(wf=wfs[i], logw=logw[i]))
+        close_log(logO[i], op.returnLogOverlap())
+        assert np.max(np.abs(gd[i] - op.returnGreenDiagonal())) < RTOL * 10
+        if mode == "dense":
+            assert np.max(np.abs(go[i] - op.returnGreenOffDiagonal())) < RTOL * 10
+        assert np.max(np.abs(f[i] - meas.getForce(hs, op, method.forceCap))) < RTOL * 10
+
+
+@pytest.mark.parametrize("decomp", ["densityCharge", "densitySpin", "hopCharge", "hopSpin"])
+def test_hs_constants_match_oracle(decomp):
+    model, om, phi, prop, ws, *_ = make_case("dense", decomp=decomp)
+    g, cd = prop.hs_constants()
+    hs = om.returnExpMinusAlphaV(0.05, decomp)
+    assert np.allclose(g, hs.gamma, rtol=1e-15, atol=1e-15)
+    assert np.allclose(cd, np.stack([hs.constDiag00, hs.constDiag01, hs.constDiag10, hs.constDiag11]), rtol=1e-15)
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_onebody_apply(mode, which):
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode)
+    prop.onebody(ws, which)
+    out, lw, al = ws.download()
+    alpha = [method.dt, method.dt * 0.5, -method.dt * 0.5][which]
+    B = om.returnExpMinusAlphaK(alpha).matrix
+    for i in range(ws.nw):
+        assert np.max(np.abs(out[i] - B @ wfs[i])) < 1e-13
+        assert lw[i] == logw[i]
+
+
+def test_hs_apply_2x2():
+    # the bandwidth-bound NiupNidnSampleSDOperation::applyToRight with general 2x2 blocks
+    model, om, phi, prop, ws, wfs, logw, method = make_case("dense")
+    rng = np.random.default_rng(5)
+    L = model.L
+    d4 = crand(rng, ws.nw, 4, L)
+    slw = crand(rng, ws.nw)
+    prop.hs_apply(ws, d4, slw)
+    out, lw, _ = ws.download()
+    for i in range(ws.nw):
+        s = NiupNidnSample(L)
+        s.diag00, s.diag01, s.diag10, s.diag11 = d4[i]
+        s.logw = slw[i]
+        ref, rlw = niupnidn_apply_right(s, wfs[i], logw[i])
+        assert np.max(np.abs(out[i] - ref)) < 1e-14
+        assert abs(lw[i] - rlw) < 1e-14
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_stabilize(mode):
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode)
+    prop.stabilize(ws)
+    out, lw, _ = ws.download()
+    for i in range(ws.nw):
+        ref = SD(wf=wfs[i], logw=logw[i])
+        ref.stabilize()
+        assert abs(lw[i] - ref.logw) < RTOL * max(1, abs(ref.logw))
+        D = out[i].conj().T @ ref.wf
+        assert np.max(np.abs(np.abs(np.diag(D)) - 1)) < 1e-10
+        assert np.max(np.abs(D - np.diag(np.diag(D)))) < 1e-10
+
+
+@pytest.mark.parametrize("mode,decomp", [("dense", "densitySpin"), ("blocks", "densitySpin"),
+                                         ("dense", "densityCharge"), ("blocks", "densityCharge"),
+                                         ("dense", "hopCharge"), ("dense", "hopSpin")])
+def test_step_matches_oracle(mode, decomp):
+    """One projectExpMinusDtKExpMinusDtV over all walkers with the same injected uniforms."""
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode, decomp=decomp, nw=10)
+    rng = np.random.default_rng(9)
+    # kill one walker up front: it must consume no uniforms and stay untouched
+    alive = np.ones(ws.nw, dtype=np.int32)
+    alive[3] = 0
+    ws.upload(wfs, logw, alive)
+    u = rng.uniform(0, 1, model.L * ws.nw)
+    prop.set_uniforms(u)
+    prop.set_ET(-7.5)
+    f_dev, aux_dev = prop.step(ws, record=True)
+    out, lw, al = ws.download()
+    om_method = OracleMethod(dt=method.dt, decompType=decomp, walkerSizePerThread=ws.nw, ET=-7.5,
+                             thermalSize=0, ETAdjustMaxSize=0)
+    drv = OracleCP(om_method, om, ListStream(u), phiT=SD(wf=phi))
+    drv.walker = [SD(wf=wfs[i], logw=logw[i]) for i in range(ws.nw)]
+    drv.walkerIsAlive = [bool(a) for a in alive]
+    drv.projectExpMinusDtKExpMinusDtV()
+    recs = {r[0]: r for r in drv.records}
+    for i in range(ws.nw):
+        assert bool(al[i]) == drv.walkerIsAlive[i]
+        if not drv.walkerIsAlive[i]:
+            continue
+        _, f_ref, aux_ref = recs[i]
+        assert np.array_equal(aux_dev[i], aux_ref)
+        assert np.max(np.abs(f_dev[i] - f_ref)) < RTOL
+        ref = drv.walker[i]
+        close_log(lw[i], ref.logw)
+        assert np.max(np.abs(out[i] - ref.wf)) < 1e-12 * max(1, np.max(np.abs(ref.wf)))
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_pop_control_matches_oracle(mode):
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode, nw=16)
+    rng = np.random.default_rng(4)
+    # one walker with a phase beyond pi/2 -> non-positive weight -> replaced
+    logw = logw.copy()
+    logw[2] += 1j * math.pi
+    ws.upload(wfs, logw)
+    u = rng.uniform(0, 1, ws.nw)
+    prop.set_uniforms(u)
+    wt, tb, st = prop.pop_control(ws, record=True)
+    om_method = OracleMethod(dt=method.dt, walkerSizePerThread=ws.nw, thermalSize=0, ETAdjustMaxSize=0)
+    drv = OracleCP(om_method, om, ListStream(u), phiT=SD(wf=phi))
+    drv.walker = [SD(wf=wfs[i], logw=logw[i]) for i in range(ws.nw)]
+    drv.walkerIsAlive = [True] * ws.nw
+    wref, tref = drv.popControl()
+    assert np.allclose(wt, wref, rtol=RTOL, atol=0)
+    assert wt[2] == 0.0
+    assert np.array_equal(tb, tref)
+    assert st == pytest.approx(ocomb.popCheck(list(wref)), rel=1e-12)
+    out, lw, al = ws.download()
+    assert np.all(al == 1)
+    for i in range(ws.nw):
+        close_log(lw[i], drv.walker[i].logw)
+        assert np.max(np.abs(out[i] - drv.walker[i].wf)) < 1e-13
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_measure_matches_oracle(mode):
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode, U=3.0, t2=-0.2)
+    # keep the overlaps O(1)
+    logw = np.zeros(ws.nw, dtype=complex)
+    ws.upload(wfs, logw)
+    prop.reset_measure()
+    prop.measure(ws)
+    e = prop.energy()
+    om_method = OracleMethod(dt=method.dt, walkerSizePerThread=ws.nw, thermalSize=0, ETAdjustMaxSize=0)
+    drv = OracleCP(om_method, om, UniformStream(1), phiT=SD(wf=phi))
+    drv.walker = [SD(wf=wfs[i], logw=logw[i]) for i in range(ws.nw)]
+    drv.walkerIsAlive = [True] * ws.nw
+    drv.addMixedMeasurement()
+    mm = drv.mixedMeasure
+    for key, ref in [("den", mm.den), ("HNum", mm.HNum), ("KNum", mm.KNum), ("VNum", mm.VNum), ("RNum", mm.RNum)]:
+        assert abs(e[key] - ref) < RTOL * max(1.0, abs(ref)), (key, e[key], ref)
+    sites = prop.site_measure()
+    assert np.allclose(sites[0], mm.NupNum, rtol=RTOL, atol=1e-12)
+    assert np.allclose(sites[1], mm.NdnNum, rtol=RTOL, atol=1e-12)
+    if mode == "dense":
+        assert np.allclose(sites[2], mm.SplusNum, rtol=RTOL, atol=1e-12)
+        assert np.allclose(sites[3], mm.SminusNum, rtol=RTOL, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_driver_trajectory_matches_oracle(mode):
+    """A whole small CP run (QR, comb, ET adjust, measurement) on the same uniform stream:
+    per-block HNum/den agree to 1e-9 relative and the uniform counters agree exactly."""
+    nx, ny, nup, ndn, U = 4, 4, 5, 5, 4.0
+    model = HubbardModel.square(nx, ny, nup + ndn, U)
+    om = HubbardSOC(model.L, model.N, model.K, U=model.U)
+    phi = free_electron_trial(model) if mode == "dense" else spin_block_trial(model, nup, ndn)
+    kw = dict(spin_blocks=False) if mode == "dense" else dict(nup=nup, spin_blocks=True)
+    m = dict(dt=0.01, thermalSize=20, writeNumber=4, measureNumberPerWrite=2, measureSkipStep=5,
+             walkerSizePerThread=16, decompType="densitySpin", mgsStep=10, popControlStep=10,
+             ETAdjustStep=10, ETAdjustMaxSize=20, seed=2024)
+    dev = AfqmcConstraintPath(ctx(), model, DevMethod(**m), phi, **kw)
+    dev.run()
+    ref = OracleCP(OracleMethod(**m), om, UniformStream(2024), phiT=SD(wf=phi))
+    ref.run()
+    assert dev.prop.counter() == ref.rng.counter
+    assert len(dev.energies) == len(ref.energies)
+    for a, b in zip(dev.energies, ref.energies):
+        for x, y in zip(a[:2], b[:2]):
+            assert abs(x - y) < 1e-9 * abs(y), (x, y)

@@ -1,0 +1,104 @@
+"""Device building blocks vs the oracle / numpy: MFMA GEMM operand layouts,
+Gauss-Jordan inverse + log-determinant, Cholesky inverse, bit-exact comb."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import comb as ocomb  # noqa: E402
+from oracle import linalg  # noqa: E402
+from tests.gpu_util import chol_inverse, crand, ctx, gj_inverse, zgemm  # noqa: E402
+
+TOL = {0: 1e-13, 1: 2e-5}
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("opa", [0, 1, 2])
+@pytest.mark.parametrize("shape", [(16, 16, 4), (37, 45, 29), (64, 64, 16), (130, 70, 97)])
+def test_zgemm_matches_numpy(precision, opa, shape):
+    rng = np.random.default_rng(sum(shape) + 7 * opa + precision)
+    M, N, K = shape
+    A = crand(rng, M, K) if opa == 0 else crand(rng, K, M)
+    B = crand(rng, K, N)
+    C = zgemm(opa, A, B, precision)[0]
+    opA = A if opa == 0 else (A.T if opa == 1 else A.conj().T)
+    ref = opA @ B
+    assert np.max(np.abs(C - ref)) < TOL[precision] * np.sqrt(K) * 4
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_zgemm_batched_strided_and_scaled(precision):
+    rng = np.random.default_rng(3)
+    batch, M, N, K = 5, 33, 20, 40
+    A = crand(rng, batch, M, K)
+    B = crand(rng, batch, K, N)
+    C = zgemm(0, A, B, precision)
+    for b in range(batch):
+        assert np.max(np.abs(C[b] - A[b] @ B[b])) < TOL[precision] * 30
+    # shared A, per column-group row scaling (the fused HS diagonal)
+    A2 = crand(rng, M, K)
+    groups = 4
+    s = crand(rng, batch, groups, K)
+    C = zgemm(0, A2, B, precision, scale=s.reshape(batch * groups, K), scale_group=N // groups)
+    for b in range(batch):
+        for g in range(groups):
+            cols = slice(g * (N // groups), (g + 1) * (N // groups))
+            ref = A2 @ (s[b, g][:, None] * B[b][:, cols])
+            assert np.max(np.abs(C[b][:, cols] - ref)) < TOL[precision] * 30
+
+
+@pytest.mark.parametrize("N", [1, 3, 10, 16, 17, 40, 64, 65, 112, 128])
+def test_gj_inverse_and_logdet(N):
+    rng = np.random.default_rng(N)
+    batch = 6
+    A = crand(rng, batch, N, N) + 0.5 * np.eye(N)
+    inv, ld, info = gj_inverse(A)
+    for b in range(batch):
+        assert info[b] == 0
+        assert np.max(np.abs(inv[b] @ A[b] - np.eye(N))) < 1e-11 * N
+        ref = linalg.log_determinant(*linalg.lu_construct(A[b]))
+        assert abs(ld[b].real - ref.real) < 1e-11 * max(1, abs(ref.real))
+        assert abs(np.exp(1j * (ld[b].imag - ref.imag)) - 1) < 1e-11
+        assert -np.pi < ld[b].imag <= np.pi
+
+
+def test_gj_pivoting_cases():
+    # permutation matrix (one swap -> det -1), exactly singular matrix, identity
+    P = np.eye(8, dtype=complex)[[1, 0, 2, 3, 4, 5, 6, 7]]
+    S = np.ones((8, 8), dtype=complex)
+    inv, ld, info = gj_inverse(np.stack([P, S, np.eye(8, dtype=complex)]))
+    assert np.allclose(inv[0], P.T)
+    assert abs(np.exp(ld[0]) - (-1)) < 1e-14
+    assert info[1] == 1 and ld[1].real == -1e300
+    assert np.allclose(inv[2], np.eye(8)) and abs(ld[2]) == 0
+
+
+@pytest.mark.parametrize("N", [5, 28, 64, 112])
+def test_chol_inverse(N):
+    rng = np.random.default_rng(100 + N)
+    batch = 4
+    W = crand(rng, batch, 3 * N, N)
+    S = np.conj(np.transpose(W, (0, 2, 1))) @ W
+    Rinv, ld, info = chol_inverse(S)
+    for b in range(batch):
+        R = np.linalg.inv(Rinv[b])
+        assert np.allclose(np.triu(R), R, atol=1e-10)
+        assert np.all(np.diag(R).real > 0)
+        assert np.max(np.abs(R.conj().T @ R - S[b])) < 1e-10 * np.max(np.abs(S[b]))
+        assert abs(ld[b].real - np.sum(np.log(np.diag(R).real))) < 1e-10 * N
+
+
+@pytest.mark.parametrize("L", [4, 100, 2048])
+def test_comb_bit_exact(L):
+    from afqmclab_amd.cp import comb
+    rng = np.random.default_rng(L)
+    for trial in range(5):
+        w = rng.uniform(0, 5, L)
+        w[rng.integers(0, L, L // 10 + 1)] = 0.0
+        u = rng.uniform(0, 1, L)
+        if trial == 4:
+            u[:] = np.nextafter(1.0, 0.0)
+        t, st = comb(ctx(), w, u)
+        ref = ocomb.popConfiguration(1, w, u)
+        assert np.array_equal(t, ref)
+        assert st == ocomb.popCheck(list(w))
