@@ -37,6 +37,8 @@ SIGNATURES = {
     "afqmc_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "afqmc_ctx_destroy": (C.c_int, [_vp]),
     "afqmc_ctx_synchronize": (C.c_int, [_vp]),
+    "afqmc_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
+    "afqmc_ctx_get_timing": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int]),
     "afqmc_cp_create": (C.c_int, [_vp, C.POINTER(CPDesc), C.POINTER(_vp)]),
     "afqmc_cp_destroy": (C.c_int, [_vp]),
     "afqmc_cp_get_hs_constants": (C.c_int, [_vp, _dp]),
@@ -47,6 +49,7 @@ SIGNATURES = {
     "afqmc_walkers_download": (C.c_int, [_vp, _dp, _dp, _ip]),
     "afqmc_cp_set_seed": (C.c_int, [_vp, C.c_uint64, C.c_uint64]),
     "afqmc_cp_get_counter": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "afqmc_cp_get_walker_steps": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
     "afqmc_cp_set_uniforms": (C.c_int, [_vp, _dp, C.c_int64]),
     "afqmc_cp_skip_uniforms": (C.c_int, [_vp, C.c_uint64]),
     "afqmc_cp_set_ET": (C.c_int, [_vp, C.c_double]),

@@ -29,6 +29,21 @@ class Context:
     def synchronize(self):
         check(lib().afqmc_ctx_synchronize(self.h))
 
+    REGIONS = ["propagate", "overlap", "gauss_jordan", "green", "hs_sample", "qr", "comb", "measure",
+               "ph_force", "ph_sample", "ph_taylor", "hs_apply"]
+
+    def set_timing(self, enable=True):
+        """Bracket tagged kernel launches with HIP events on this context's stream (resets totals)."""
+        check(lib().afqmc_ctx_set_timing(self.h, int(bool(enable))))
+
+    def timing(self):
+        """{region: (total_ms, launches)} since set_timing."""
+        n = len(self.REGIONS)
+        ms = np.zeros(n)
+        cnt = (C.c_longlong * n)()
+        check(lib().afqmc_ctx_get_timing(self.h, dptr(ms), cnt, n))
+        return {r: (float(ms[i]), int(cnt[i])) for i, r in enumerate(self.REGIONS)}
+
     def close(self):
         if self.h:
             lib().afqmc_ctx_destroy(self.h)
@@ -169,6 +184,11 @@ class CPPropagator:
     def set_uniforms(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         check(lib().afqmc_cp_set_uniforms(self.h, dptr(u) if u.size else None, u.size))
+
+    def walker_steps(self):
+        c = C.c_uint64()
+        check(lib().afqmc_cp_get_walker_steps(self.h, C.byref(c)))
+        return c.value
 
     def skip_uniforms(self, n):
         check(lib().afqmc_cp_skip_uniforms(self.h, C.c_uint64(int(n))))

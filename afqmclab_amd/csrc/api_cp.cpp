@@ -250,30 +250,46 @@ HsSampleArgs hs_args(afqmc_cp cp, afqmc_walkers w) {
 template <typename T>
 int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out) {
   hipStream_t s = cp->ctx->stream;
-  int rc = do_overlap<T>(cp, w, true);
-  if (rc) return rc;
+  int rc;
+  {
+    RegionTimer t(cp->ctx, REG_OVERLAP);
+    if ((rc = overlap_gemm<T>(cp, w, w->wf))) return rc;
+  }
+  {
+    RegionTimer t(cp->ctx, REG_GJ);
+    if ((rc = gj_blocks<T>(cp, w, true, w->alive.p))) return rc;
+  }
   LogOArgs l = logo_args<T>(cp, w, LOGO_STEP);
   if ((rc = launch_logo(l, s))) return rc;
   if ((rc = launch_alive_scan(w->nw, w->alive.p, cp->prefix.p, cp->sc.p, s))) return rc;
-  if ((rc = green_gemm<T>(cp, w, w->wf, w->alive.p, false))) return rc;
+  {
+    RegionTimer t(cp->ctx, REG_GREEN);
+    if ((rc = green_gemm<T>(cp, w, w->wf, w->alive.p, false))) return rc;
+  }
   HsSampleArgs h = hs_args<T>(cp, w);
   if (force_out || aux_out) {
     h.force_out = cp->force_rec.p;
     h.aux_out = cp->aux_rec.p;
   }
-  if ((rc = launch_hs_sample(h, s))) return rc;
-  if ((rc = launch_advance_counter(cp->sc.p, cp->L, s))) return rc;
+  {
+    RegionTimer t(cp->ctx, REG_HS_SAMPLE);
+    if ((rc = launch_hs_sample(h, s))) return rc;
+    if ((rc = launch_advance_counter(cp->sc.p, cp->L, s))) return rc;
+  }
   if (cp->diag_only) {
+    RegionTimer t(cp->ctx, REG_PROPAGATE);
     if ((rc = propagate_gemm<T>(cp, w, 0, w->wf, w->tmp, true))) return rc;
   } else {
-    if ((rc = launch_hs_apply_2x2<T>(w->nw, cp->L, cp->N, w->wf[0].p, w->tmp[0].p, cp->d2x2.p, s))) return rc;
-    if ((rc = propagate_gemm<T>(cp, w, 0, w->tmp, w->wf, false))) return rc;
-    // result already in wf
-    if (force_out || aux_out) goto records;
-    return 0;
+    {
+      RegionTimer t(cp->ctx, REG_HS_APPLY);
+      if ((rc = launch_hs_apply_2x2<T>(w->nw, cp->L, cp->N, w->wf[0].p, w->tmp[0].p, cp->d2x2.p, s))) return rc;
+    }
+    for (int b = 0; b < cp->nblk; ++b) std::swap(w->wf[b], w->tmp[b]);
+    RegionTimer t(cp->ctx, REG_PROPAGATE);
+    if ((rc = propagate_gemm<T>(cp, w, 0, w->wf, w->tmp, false))) return rc;
   }
+  // the propagated walkers are in tmp: make them current
   for (int b = 0; b < cp->nblk; ++b) std::swap(w->wf[b], w->tmp[b]);
-records:
   if (force_out) HIP_TRY(hipMemcpyAsync(force_out, cp->force_rec.p, sizeof(double) * w->nw * cp->L, hipMemcpyDeviceToHost, s));
   if (aux_out) HIP_TRY(hipMemcpyAsync(aux_out, cp->aux_rec.p, sizeof(int) * w->nw * cp->L, hipMemcpyDeviceToHost, s));
   if (force_out || aux_out) HIP_TRY(hipStreamSynchronize(s));
@@ -315,6 +331,7 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
 }
 
 int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
+  RegionTimer t(cp->ctx, REG_QR);
   if (!cp->fp32) return qr_impl_f64(cp, w, w->wf, w->tmp);
   // mixed path: QR in FP64 on a widened copy of the FP32 walkers
   hipStream_t s = cp->ctx->stream;
@@ -332,6 +349,7 @@ int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
 template <typename T>
 int pop_control_impl(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats) {
   hipStream_t s = cp->ctx->stream;
+  RegionTimer t(cp->ctx, REG_COMB);
   int rc = do_overlap<T>(cp, w, false);
   if (rc) return rc;
   LogOArgs l = logo_args<T>(cp, w, LOGO_WEIGHT);
@@ -356,6 +374,7 @@ int pop_control_impl(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* tab
 template <typename T>
 int measure_impl(afqmc_cp cp, afqmc_walkers w) {
   hipStream_t s = cp->ctx->stream;
+  RegionTimer t(cp->ctx, REG_MEASURE);
   int rc;
   if ((rc = propagate_gemm<T>(cp, w, 2, w->wf, w->tmp, false))) return rc;
   if ((rc = launch_logw_shift(w->nw, w->logw.p, w->logw_tmp.p, cp->hop_logw[2], s))) return rc;
@@ -691,6 +710,16 @@ int afqmc_cp_get_counter(afqmc_cp cp, uint64_t* counter) {
   DevScalars h;
   HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
   *counter = h.counter;
+  return 0;
+}
+
+int afqmc_cp_get_walker_steps(afqmc_cp cp, uint64_t* walker_steps) {
+  REQUIRE(cp && walker_steps, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  DevScalars h;
+  HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
+  *walker_steps = h.walker_steps;
   return 0;
 }
 

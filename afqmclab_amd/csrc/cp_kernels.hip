@@ -255,6 +255,7 @@ __global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
 
 __global__ void advance_counter_kernel(DevScalars* sc, int per_walker) {
   sc->counter += (unsigned long long)per_walker * (unsigned long long)sc->alive_count;
+  sc->walker_steps += (unsigned long long)sc->alive_count;
 }
 
 // ------------------------------------------------------------------ HS 2x2 apply

@@ -16,6 +16,7 @@ struct DevScalars {
   zd NupTot, NdnTot, SplusTot, SminusTot;
   double wmin, wmax, wavg;     // popCheck of the last comb
   double taylor_orders;        // phaseless: total Taylor terms applied in the last step
+  unsigned long long walker_steps;  // walkers propagated, summed over steps (alive walkers only)
 };
 
 __device__ __forceinline__ double afqmc_uniform_dev(unsigned long long key, unsigned long long k) {

@@ -148,4 +148,37 @@ int afqmc_ctx_synchronize(afqmc_ctx ctx) {
   return 0;
 }
 
+static int resolve_events(afqmc_ctx ctx) {
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (auto& e : ctx->events) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+    ctx->total_ms[e.first] += ms;
+    ctx->count[e.first] += 1;
+    hipEventDestroy(e.second.first);
+    hipEventDestroy(e.second.second);
+  }
+  ctx->events.clear();
+  return 0;
+}
+
+int afqmc_ctx_set_timing(afqmc_ctx ctx, int enable) {
+  REQUIRE(ctx, "null argument");
+  ScopedDevice sd(ctx);
+  int rc = resolve_events(ctx);
+  if (rc) return rc;
+  ctx->timing = enable ? 1 : 0;
+  for (int r = 0; r < REG_COUNT; ++r) { ctx->total_ms[r] = 0.0; ctx->count[r] = 0; }
+  return 0;
+}
+
+int afqmc_ctx_get_timing(afqmc_ctx ctx, double* total_ms, long long* count, int n) {
+  REQUIRE(ctx && total_ms && count, "null argument");
+  ScopedDevice sd(ctx);
+  int rc = resolve_events(ctx);
+  if (rc) return rc;
+  for (int r = 0; r < n && r < REG_COUNT; ++r) { total_ms[r] = ctx->total_ms[r]; count[r] = ctx->count[r]; }
+  return 0;
+}
+
 }  // extern "C"

@@ -3,14 +3,45 @@
 #include <hip/hip_runtime.h>
 
 #include <utility>
+#include <vector>
 
 #include "../../include/afqmc_mi355x.h"
 #include "common.hpp"
 #include "cp_kernels.hpp"
 
+// Optional per-region kernel timing: when enabled, tagged launches are bracketed
+// by HIP events on the context stream; totals are resolved on query.
+enum TimedRegion {
+  REG_PROPAGATE = 0, REG_OVERLAP, REG_GJ, REG_GREEN, REG_HS_SAMPLE, REG_QR, REG_COMB, REG_MEASURE, REG_PH_FORCE,
+  REG_PH_SAMPLE, REG_PH_TAYLOR, REG_HS_APPLY, REG_COUNT
+};
 struct afqmc_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  int timing = 0;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
+  double total_ms[REG_COUNT] = {0};
+  long long count[REG_COUNT] = {0};
+};
+
+// RAII bracket of one tagged region on the context stream
+struct RegionTimer {
+  afqmc_ctx ctx;
+  int reg;
+  hipEvent_t a = nullptr, b = nullptr;
+  RegionTimer(afqmc_ctx c, int r) : ctx(c), reg(r) {
+    if (ctx && ctx->timing) {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~RegionTimer() {
+    if (a) {
+      hipEventRecord(b, ctx->stream);
+      ctx->events.push_back({reg, {a, b}});
+    }
+  }
 };
 
 // Selects the context's device for the lifetime of a C entry point.

@@ -33,6 +33,14 @@ int afqmc_version(void);
 int afqmc_ctx_create(int device, afqmc_ctx* out);
 int afqmc_ctx_destroy(afqmc_ctx ctx);
 int afqmc_ctx_synchronize(afqmc_ctx ctx);
+/* Per-region kernel timing with HIP events on the context stream (no reference counterpart;
+ * the reference only has TimerHao wall time, time_hao.h:16-41).  Regions, in order:
+ * 0 propagate (one-body GEMM with fused HS scaling), 1 overlap GEMM, 2 Gauss-Jordan,
+ * 3 Green GEMM, 4 HS sampling, 5 QR, 6 comb, 7 measurement, 8 phaseless force bias,
+ * 9 phaseless sample matrix, 10 phaseless Taylor, 11 HS 2x2 apply.
+ * set_timing resets the totals; get_timing synchronises and returns n totals (ms) and counts. */
+int afqmc_ctx_set_timing(afqmc_ctx ctx, int enable);
+int afqmc_ctx_get_timing(afqmc_ctx ctx, double* total_ms, long long* count, int n);
 
 /* ------------------------------------------------------------------ CP (Hubbard) */
 /* Model + method description.  Mirrors HubbardSOC (HubbardSOC.cpp:39-59), the
@@ -85,6 +93,8 @@ int afqmc_walkers_download(afqmc_walkers w, double* wf, double* logw, int* alive
  * uniforms in the reference's consumption order (uniformHao, random_hao.cpp:73-76) */
 int afqmc_cp_set_seed(afqmc_cp cp, uint64_t seed, uint64_t counter);
 int afqmc_cp_get_counter(afqmc_cp cp, uint64_t* counter);
+/* walkers propagated by afqmc_cp_step, summed over steps (alive walkers only) */
+int afqmc_cp_get_walker_steps(afqmc_cp cp, uint64_t* walker_steps);
 int afqmc_cp_set_uniforms(afqmc_cp cp, const double* u, int64_t n); /* n = 0: back to the device stream */
 /* consume n uniforms without using them (e.g. the aux sample drawn by estimateMemory,
  * afqmcConstraintPath.cpp:66) */
