@@ -30,6 +30,17 @@ class CPDesc(C.Structure):
     ]
 
 
+class PhDesc(C.Structure):
+    _fields_ = [
+        ("L", C.c_int), ("Nup", C.c_int), ("Ndn", C.c_int), ("G", C.c_int), ("precision", C.c_int),
+        ("force_type", C.c_int), ("dt", C.c_double), ("force_cap", C.c_double),
+        ("t", _dp), ("chol", _dp), ("bg", _dp), ("phiT_up", _dp), ("phiT_dn", _dp),
+        ("phiT_logw", C.c_double * 2), ("half_rot_up", _dp), ("half_rot_dn", _dp), ("const_force", _dp),
+        ("max_walkers", C.c_int), ("taylor_fixed", C.c_int), ("taylor_order", C.c_int), ("taylor_base", C.c_int),
+        ("taylor_accuracy", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "afqmc_last_error": (C.c_char_p, []),
@@ -69,6 +80,33 @@ SIGNATURES = {
     "afqmc_cp_site_measure": (C.c_int, [_vp, _dp]),
     "afqmc_cp_reset_measure": (C.c_int, [_vp]),
     "afqmc_cp_adjust_ET": (C.c_int, [_vp]),
+    "afqmc_ph_create": (C.c_int, [_vp, C.POINTER(PhDesc), C.POINTER(_vp)]),
+    "afqmc_ph_destroy": (C.c_int, [_vp]),
+    "afqmc_ph_set_onebody": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int]),
+    "afqmc_ph_set_background": (C.c_int, [_vp, _dp]),
+    "afqmc_ph_walkers_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "afqmc_ph_walkers_destroy": (C.c_int, [_vp]),
+    "afqmc_ph_walkers_upload": (C.c_int, [_vp, _dp, _dp, _ip]),
+    "afqmc_ph_walkers_download": (C.c_int, [_vp, _dp, _dp, _ip]),
+    "afqmc_ph_set_seed": (C.c_int, [_vp, C.c_uint64, C.c_uint64]),
+    "afqmc_ph_get_counter": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "afqmc_ph_set_uniforms": (C.c_int, [_vp, _dp, C.c_int64]),
+    "afqmc_ph_skip_uniforms": (C.c_int, [_vp, C.c_uint64]),
+    "afqmc_ph_set_ET": (C.c_int, [_vp, C.c_double]),
+    "afqmc_ph_get_ET": (C.c_int, [_vp, _dp]),
+    "afqmc_ph_log_overlap": (C.c_int, [_vp, _vp, _dp]),
+    "afqmc_ph_background": (C.c_int, [_vp, _vp, _dp]),
+    "afqmc_ph_sample_apply": (C.c_int, [_vp, _vp, _dp, _dp, _dp, _ip]),
+    "afqmc_ph_sample_matrix": (C.c_int, [_vp, _vp, _dp]),
+    "afqmc_ph_taylor_orders": (C.c_int, [_vp, _vp, _ip]),
+    "afqmc_ph_onebody": (C.c_int, [_vp, _vp, C.c_int]),
+    "afqmc_ph_stabilize": (C.c_int, [_vp, _vp]),
+    "afqmc_ph_step": (C.c_int, [_vp, _vp, _dp, _dp]),
+    "afqmc_ph_pop_control": (C.c_int, [_vp, _vp, _dp, _ip, _dp]),
+    "afqmc_ph_measure": (C.c_int, [_vp, _vp]),
+    "afqmc_ph_energy": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "afqmc_ph_reset_measure": (C.c_int, [_vp]),
+    "afqmc_ph_adjust_ET": (C.c_int, [_vp]),
     "afqmc_test_zgemm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_longlong,
                                    _dp, _dp, C.c_int, _dp]),
     "afqmc_test_gj_inverse": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip]),

@@ -710,6 +710,7 @@ int afqmc_cp_get_counter(afqmc_cp cp, uint64_t* counter) {
   DevScalars h;
   HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
   *counter = h.counter;
+  REQUIRE(!h.uniform_overrun, "the host-provided uniforms were exhausted (%llu given)", h.ulen);
   return 0;
 }
 
@@ -738,6 +739,8 @@ int afqmc_cp_set_uniforms(afqmc_cp cp, const double* u, int64_t n) {
   DevScalars h;
   HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
   h.counter = 0;
+  h.ulen = (unsigned long long)n;
+  h.uniform_overrun = 0;
   HIP_TRY(hipMemcpy(cp->sc.p, &h, sizeof(h), hipMemcpyHostToDevice));
   return 0;
 }
@@ -914,6 +917,7 @@ int afqmc_comb(afqmc_ctx ctx, int nw, const double* weight, const double* unifor
   if (rc) return rc;
   DevScalars h;
   memset(&h, 0, sizeof(h));
+  h.ulen = (unsigned long long)nw;
   HIP_TRY(hipMemcpy(sc.p, &h, sizeof(h), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(w.p, weight, sizeof(double) * nw, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(u.p, uniforms, sizeof(double) * nw, hipMemcpyHostToDevice));

@@ -21,16 +21,23 @@ template <typename T>
 int zgemm_t(afqmc_ctx ctx, int opa, int M, int N, int K, int batch, const double* A, long long strideA, const double* B,
             const double* scale, int scale_group, double* C) {
   hipStream_t s = ctx->stream;
-  const long long lda = (opa == OP_N) ? M : K;
+  const long long lda = (opa == OP_N || opa == OP_NR) ? M : K;
   const size_t nA = (size_t)(strideA ? strideA * batch : (long long)M * K);
   const size_t nB = (size_t)K * N * batch, nC = (size_t)M * N * batch;
-  auto hA = narrow<T>(A, nA), hB = narrow<T>(B, nB);
+  auto hB = narrow<T>(B, nB);
   DevBuf<cplx<T>> dA, dB, dC, dS;
   int rc = dA.alloc(nA);
   if (!rc) rc = dB.alloc(nB);
   if (!rc) rc = dC.alloc(nC);
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(dA.p, hA.data(), sizeof(cplx<T>) * nA, hipMemcpyHostToDevice));
+  if (opa == OP_NR) {  // A holds nA real values
+    std::vector<T> hr(nA);
+    for (size_t i = 0; i < nA; ++i) hr[i] = (T)A[i];
+    HIP_TRY(hipMemcpy(dA.p, hr.data(), sizeof(T) * nA, hipMemcpyHostToDevice));
+  } else {
+    auto hA = narrow<T>(A, nA);
+    HIP_TRY(hipMemcpy(dA.p, hA.data(), sizeof(cplx<T>) * nA, hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipMemcpy(dB.p, hB.data(), sizeof(cplx<T>) * nB, hipMemcpyHostToDevice));
   GemmArgs a;
   memset(&a, 0, sizeof(a));
@@ -94,8 +101,9 @@ int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K,
                      long long strideA, const double* B, const double* scale, int scale_group, double* C) {
   REQUIRE(ctx && A && B && C, "null argument");
   REQUIRE(M > 0 && N > 0 && K > 0 && batch > 0, "sizes must be positive");
-  REQUIRE(opa >= 0 && opa <= 2, "opa must be 0 (N), 1 (T) or 2 (C)");
+  REQUIRE(opa >= 0 && opa <= 3, "opa must be 0 (N), 1 (T), 2 (C) or 3 (N with real A)");
   REQUIRE(!scale || opa == OP_N, "row scaling is supported with op(A) = N");
+  REQUIRE(opa != OP_NR || !strideA || strideA > 0, "invalid strideA");
   ScopedDevice sd(ctx);
   return precision ? zgemm_t<float>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, C)
                    : zgemm_t<double>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, C);

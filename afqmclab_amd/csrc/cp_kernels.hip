@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
       if (f < -a.cap) f = -a.cap;
     }
     const unsigned long long k = base + (unsigned long long)i;
-    const double u = a.ubuf ? a.ubuf[k] : afqmc_uniform_dev(key, k);
+    const double u = afqmc_take_uniform(a.ubuf, a.sc, key, k);
     const double em = exp(-f), ep = exp(f);
     const double prob = em / (em + ep);
     const int x = (u < prob) ? -1 : 1;
@@ -301,7 +301,7 @@ __global__ void comb_kernel(int L, const double* weight, const double* ubuf, Dev
   double prob_sum_old = prob[0];
   for (int i = 0; i < L; ++i) {
     const unsigned long long k = c0 + (unsigned long long)i;
-    const double u = ubuf ? ubuf[k] : afqmc_uniform_dev(key, k);
+    const double u = afqmc_take_uniform(ubuf, sc, key, k);
     const double prob_sum_new = (i + u) / (L * 1.0);
     while (prob_sum_new > prob_sum_old) {
       if (index_old + 1 >= L) break;  // the reference has no guard here

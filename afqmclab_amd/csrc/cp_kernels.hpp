@@ -11,12 +11,13 @@ struct DevScalars {
   unsigned long long key;      // uniform stream key
   unsigned long long counter;  // next uniform index
   int alive_count;
-  int pad;
+  int uniform_overrun;         // set when a kernel asked for a host-provided uniform beyond ulen
   zd den, HNum, KNum, VNum, RNum;
   zd NupTot, NdnTot, SplusTot, SminusTot;
   double wmin, wmax, wavg;     // popCheck of the last comb
   double taylor_orders;        // phaseless: total Taylor terms applied in the last step
   unsigned long long walker_steps;  // walkers propagated, summed over steps (alive walkers only)
+  unsigned long long ulen;     // number of host-provided uniforms (when a uniform buffer is set)
 };
 
 __device__ __forceinline__ double afqmc_uniform_dev(unsigned long long key, unsigned long long k) {
@@ -27,6 +28,16 @@ __device__ __forceinline__ double afqmc_uniform_dev(unsigned long long key, unsi
   return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 unsigned long long afqmc_seed_mix_host(unsigned long long seed);
+
+// uniform number k of the run: from the host-provided list when one is set (reads stay inside
+// the list; an index past its end flags uniform_overrun and yields 0.5), else from the stream
+__device__ __forceinline__ double afqmc_take_uniform(const double* ubuf, DevScalars* sc, unsigned long long key,
+                                                     unsigned long long k) {
+  if (!ubuf) return afqmc_uniform_dev(key, k);
+  if (k < sc->ulen) return ubuf[k];
+  sc->uniform_overrun = 1;
+  return 0.5;
+}
 
 // logO combination of per-block log-determinants
 enum { COMBINE_WRAP = 0, COMBINE_SUM = 1 };

@@ -34,6 +34,7 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
 
   const ct* __restrict__ A = reinterpret_cast<const ct*>(p.A) + (long long)b * p.strideA;
+  const T* __restrict__ Ar = reinterpret_cast<const T*>(p.A) + (long long)b * p.strideA;  // OP_NR
   const ct* __restrict__ Bm = reinterpret_cast<const ct*>(p.B) + (long long)b * p.strideB;
   const ct* __restrict__ S = SCALE ? reinterpret_cast<const ct*>(p.scale) + (long long)b * p.scale_strideB : nullptr;
 
@@ -53,12 +54,13 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
     for (int e = 0; e < A_ELEMS; ++e) {
       int idx = tid + e * NT;
       int m, k;
-      if (OPA == OP_N) { m = idx % BM; k = idx / BM; }
+      if (OPA == OP_N || OPA == OP_NR) { m = idx % BM; k = idx / BM; }
       else { k = idx % BK; m = idx / BK; }
       int gm = m0 + m, gk = k0 + k;
       ct v = cmk<T>(0, 0);
       if (gm < p.M && gk < p.K) {
-        v = (OPA == OP_N) ? A[gm + (long long)gk * p.lda] : A[gk + (long long)gm * p.lda];
+        if (OPA == OP_NR) v = cmk<T>(Ar[gm + (long long)gk * p.lda], (T)0);
+        else v = (OPA == OP_N) ? A[gm + (long long)gk * p.lda] : A[gk + (long long)gm * p.lda];
       }
       ra[e] = v;
     }
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
     for (int e = 0; e < A_ELEMS; ++e) {
       int idx = tid + e * NT;
       int m, k;
-      if (OPA == OP_N) { m = idx % BM; k = idx / BM; }
+      if (OPA == OP_N || OPA == OP_NR) { m = idx % BM; k = idx / BM; }
       else { k = idx % BK; m = idx / BK; }
       As[0][k][m] = ra[e].re;
       As[1][k][m] = (OPA == OP_C) ? -ra[e].im : ra[e].im;
@@ -125,9 +127,11 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           acc_re[i][j] = mfma_traits<T>::mma(are[i], bre[j], acc_re[i][j]);
-          acc_re[i][j] = mfma_traits<T>::mma(naim[i], bim[j], acc_re[i][j]);
           acc_im[i][j] = mfma_traits<T>::mma(are[i], bim[j], acc_im[i][j]);
-          acc_im[i][j] = mfma_traits<T>::mma(aim[i], bre[j], acc_im[i][j]);
+          if (OPA != OP_NR) {  // real A: the two products with Im(A) vanish
+            acc_re[i][j] = mfma_traits<T>::mma(naim[i], bim[j], acc_re[i][j]);
+            acc_im[i][j] = mfma_traits<T>::mma(aim[i], bre[j], acc_im[i][j]);
+          }
         }
     }
     __syncthreads();
@@ -271,6 +275,7 @@ int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s) {
     if (opa == OP_N) return sc ? dispatch_tile<T, OP_N, EPI_STORE, true>(a, s) : dispatch_tile<T, OP_N, EPI_STORE, false>(a, s);
     if (opa == OP_T) return dispatch_tile<T, OP_T, EPI_STORE, false>(a, s);
     if (opa == OP_C) return dispatch_tile<T, OP_C, EPI_STORE, false>(a, s);
+    if (opa == OP_NR) return dispatch_tile<T, OP_NR, EPI_STORE, false>(a, s);
   } else {
     if (opa == OP_N) return dispatch_tile<T, OP_N, EPI_GREEN, false>(a, s);
   }

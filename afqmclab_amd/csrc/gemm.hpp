@@ -10,7 +10,7 @@
 #pragma once
 #include "common.hpp"
 
-enum { OP_N = 0, OP_T = 1, OP_C = 2 };
+enum { OP_N = 0, OP_T = 1, OP_C = 2, OP_NR = 3 };  // OP_NR: A real (element type T), op N
 enum { EPI_STORE = 0, EPI_GREEN = 1 };
 
 struct GemmArgs {

@@ -100,7 +100,7 @@ int launch_select_alive(int nw, long long n, size_t esz, const int* alive, const
   return 0;
 }
 int launch_reset_measure(DevScalars* sc, zd* site_acc, int L, hipStream_t s) {
-  hipLaunchKernelGGL(reset_measure_kernel, dim3((4 * L + 255) / 256), dim3(256), 0, s, sc, site_acc, L);
+  hipLaunchKernelGGL(reset_measure_kernel, dim3((4 * L + 255) / 256 + (L == 0)), dim3(256), 0, s, sc, site_acc, L);
   HIP_LAUNCH_CHECK();
   return 0;
 }

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(TG * TG) void gj_kernel(SmallDenseArgs p) {
   ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
 
   __shared__ zd colbuf[2][NMAX];
-  __shared__ zd rowbuf[2][NMAX];
+  __shared__ zd pivs[NMAX];
   __shared__ int perm[NMAX];
   __shared__ int qinv[NMAX];
   __shared__ unsigned char seen[NMAX];
@@ -51,8 +51,6 @@ __global__ __launch_bounds__(TG * TG) void gj_kernel(SmallDenseArgs p) {
     }
   // rows handled by this lane in the pivot search: lane, lane+64, lane+128
   unsigned used = 0u;
-  double lognorm = 0.0;
-  zd phase = cmk<double>(1.0, 0.0);
   int singular = 0;
 
   for (int k = 0; k < N; ++k) {
@@ -68,27 +66,30 @@ __global__ __launch_bounds__(TG * TG) void gj_kernel(SmallDenseArgs p) {
       }
     }
     __syncthreads();
-    // pivot search (redundant in every wave)
-    double bv = -1.0;
-    int bi = 0x7fffffff;
+    // pivot search (redundant in every wave).  Key = bits of the non-negative score (monotone
+    // as an unsigned integer) with its low 8 bits replaced by 255 - row: the maximum key is the
+    // largest score, ties (to 2^-44 relative) going to the smallest row, as izamax does.
+    unsigned long long key = 0ull;
 #pragma unroll
     for (int s = 0; s < (NMAX + 63) / 64; ++s) {
       int i = lane + 64 * s;
       if (i < N && !((used >> s) & 1u)) {
         double v = abs1(colbuf[buf][i]);
         if (!(v >= 0.0)) v = 0.0;  // NaN input: still pick a valid unused row (reported singular)
-        if (v > bv) { bv = v; bi = i; }
+        unsigned long long kk = ((unsigned long long)__double_as_longlong(v) & ~0xFFull) |
+                                (unsigned long long)(255 - i) | (1ull << 63);
+        key = kk > key ? kk : key;
       }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      double ov = __shfl_xor(bv, o, 64);
-      int oi = __shfl_xor(bi, o, 64);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      unsigned long long ok = (unsigned long long)__shfl_xor((long long)key, o, 64);
+      key = ok > key ? ok : key;
     }
-    // at step k there are N-k >= 1 unused rows and every one of them scores >= 0 > -1, so
-    // bi is a valid row; the clamp keeps every later index in range regardless
+    // bit 63 marks a candidate; at step k there are N-k >= 1 unused rows, so the key is set
+    const int bi = 255 - (int)(key & 0xFFull);
     const int pr = (bi >= 0 && bi < N) ? bi : 0;
+    const double bv = __longlong_as_double((long long)(key & ~(0xFFull | (1ull << 63))));
     if ((pr & 63) == lane) used |= 1u << (pr >> 6);
     const zd piv = colbuf[buf][pr];
     zd inv = cmk<double>(0, 0);
@@ -99,48 +100,67 @@ __global__ __launch_bounds__(TG * TG) void gj_kernel(SmallDenseArgs p) {
     }
     if (tid == 0) {
       perm[k] = pr;
-      if (!singular) {
-        double ap = cabs(piv);
-        lognorm += log(ap);
-        phase = cmul(phase, cmk<double>(piv.re / ap, piv.im / ap));
+      pivs[k] = piv;
+    }
+    // Pivot row: thread (pr % TG, tc) owns row pr for this thread's columns, and it sits in
+    // the same wave (lane pr % TG + TG * (tc % (64 / TG))), so the row travels by shuffle.
+    zd rowv[E];
+    {
+      const int ar = pr / TG;
+      const int src = (pr % TG) + TG * (tc % (64 / TG));
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        zd v = cmk<double>(0, 0);
+#pragma unroll
+        for (int a = 0; a < E; ++a)
+          if (a == ar) v = x[a][c];
+        v.re = __shfl(v.re, src, 64);
+        v.im = __shfl(v.im, src, 64);
+        const int j = tc + TG * c;
+        rowv[c] = (j == k) ? cmk<double>(0, 0) : cmul(v, inv);
       }
     }
+    // rank-1 update of every element (rowv[k] = 0 leaves column k alone; padded rows and
+    // columns beyond N compute on values that are never stored), then the pivot row and the
+    // pivot column are overwritten by the threads that own them
+    zd cv[E];
+#pragma unroll
+    for (int a = 0; a < E; ++a) cv[a] = colbuf[buf][tr + TG * a];
+#pragma unroll
+    for (int a = 0; a < E; ++a)
+#pragma unroll
+      for (int c = 0; c < E; ++c) x[a][c] = csub(x[a][c], cmul(cv[a], rowv[c]));
     if (tr == pr % TG) {
       const int ar = pr / TG;
 #pragma unroll
-      for (int a = 0; a < E; ++a) {
-        if (a != ar) continue;
+      for (int a = 0; a < E; ++a)
+        if (a == ar)
 #pragma unroll
-        for (int c = 0; c < E; ++c) {
-          int j = tc + TG * c;
-          if (j < N) rowbuf[buf][j] = (j == k) ? cmk<double>(0, 0) : cmul(x[a][c], inv);
-        }
-      }
+          for (int c = 0; c < E; ++c) x[a][c] = cmk<double>(-rowv[c].re, -rowv[c].im);
     }
-    __syncthreads();
+    if (tc == k % TG) {
+      const int kc = k / TG;
 #pragma unroll
-    for (int a = 0; a < E; ++a) {
-      int i = tr + TG * a;
-      if (i >= N) continue;
-      zd ci = colbuf[buf][i];
+      for (int c = 0; c < E; ++c)
+        if (c == kc)
 #pragma unroll
-      for (int c = 0; c < E; ++c) {
-        int j = tc + TG * c;
-        if (j >= N) continue;
-        zd r = rowbuf[buf][j];
-        if (i == pr) {
-          x[a][c] = (j == k) ? inv : cmk<double>(-r.re, -r.im);
-        } else if (j == k) {
-          x[a][c] = cmul(ci, inv);
-        } else {
-          x[a][c] = csub(x[a][c], cmul(ci, r));
-        }
-      }
+          for (int a = 0; a < E; ++a) x[a][c] = (tr + TG * a == pr) ? inv : cmul(cv[a], inv);
     }
   }
   __syncthreads();
   if (tid < N) qinv[perm[tid]] = tid;
   if (tid == 0) {
+    // log|det| and the phase product in pivot order, off the elimination's critical path
+    double lognorm = 0.0;
+    zd phase = cmk<double>(1.0, 0.0);
+    if (!singular) {
+      for (int k = 0; k < N; ++k) {
+        const zd pv = pivs[k];
+        const double ap = cabs(pv);
+        lognorm += log(ap);
+        phase = cmul(phase, cmk<double>(pv.re / ap, pv.im / ap));
+      }
+    }
     // sign of the permutation k -> perm[k]
     int visited_parity = 0;
     for (int i = 0; i < N; ++i) seen[i] = 0;
@@ -251,10 +271,11 @@ __global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
       int i = tr + TG * a, j = tc + TG * c;
       if (j < i) x[a][c] = cmk<double>(0, 0);
     }
-  // R^{-1} by the exchange sweep with pivot (k, k)
+  // R^{-1} by the exchange sweep with pivot (k, k); the pivot row reaches each thread by an
+  // in-wave shuffle (see gj_kernel), so one barrier per step
+  __syncthreads();
   for (int k = 0; k < N; ++k) {
     const int buf = k & 1;
-    __syncthreads();
     if (tc == k % TG) {
       const int kc = k / TG;
 #pragma unroll
@@ -265,20 +286,23 @@ __global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
           if (c == kc && i < N) colbuf[buf][i] = x[a][c];
       }
     }
-    if (tr == k % TG) {
+    __syncthreads();
+    const zd inv = cinv(colbuf[buf][k]);
+    zd rowv[E];
+    {
       const int ar = k / TG;
+      const int src = (k % TG) + TG * (tc % (64 / TG));
 #pragma unroll
-      for (int a = 0; a < E; ++a) {
-        if (a != ar) continue;
+      for (int c = 0; c < E; ++c) {
+        zd v = cmk<double>(0, 0);
 #pragma unroll
-        for (int c = 0; c < E; ++c) {
-          int j = tc + TG * c;
-          if (j < N) rowbuf[buf][j] = x[a][c];
-        }
+        for (int a = 0; a < E; ++a)
+          if (a == ar) v = x[a][c];
+        v.re = __shfl(v.re, src, 64);
+        v.im = __shfl(v.im, src, 64);
+        rowv[c] = cmul(v, inv);
       }
     }
-    __syncthreads();
-    const zd inv = cinv(rowbuf[buf][k]);
 #pragma unroll
     for (int a = 0; a < E; ++a) {
       int i = tr + TG * a;
@@ -288,7 +312,7 @@ __global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
       for (int c = 0; c < E; ++c) {
         int j = tc + TG * c;
         if (j >= N) continue;
-        zd r = cmul(rowbuf[buf][j], inv);
+        zd r = rowv[c];
         if (i == k) {
           x[a][c] = (j == k) ? inv : cmk<double>(-r.re, -r.im);
         } else if (j == k) {

@@ -140,11 +140,99 @@ int afqmc_cp_reset_measure(afqmc_cp cp);
 /* adjustETAndResetMeasurement (afqmcConstraintPathMeasure.cpp:35-45): ET = Re(HNum/den) on device, then reset */
 int afqmc_cp_adjust_ET(afqmc_cp cp);
 
+/* ------------------------------------------------------------------ phaseless (real materials) */
+/* Model, trial and method of the phaseless driver: RealMaterialMolecule (realMaterialMolecule.h:16-40,
+ * t, choleskyVecs, choleskyBg), the SD2s trial, the half-rotated vectors of
+ * RealMaterialMoleculeMeasureFixedSD2sSD2is::initWfDaggerCholeskyVecs (:165-186), the method keys
+ * forceType / forceCap / dt (afqmcPhaselessMethod.cpp:158-185) and LogHop2isSD2isOperation's
+ * parameters (logHop2isSD2isOperation.h:20-33). */
+typedef struct afqmc_ph_walkers_s* afqmc_ph_walkers;
+typedef struct {
+  int L;                      /* orbitals; SD2is walkers are L x max(Nup, Ndn) */
+  int Nup, Ndn;
+  int G;                      /* Cholesky vectors (choleskyNumber) */
+  int precision;              /* 0: complex FP64; 1: complex FP32 walkers and GEMMs, FP64 LU, logdets, QR,
+                                 weights, sample logw and energies */
+  int force_type;             /* 0 dynamicForce, 1 constForce */
+  double dt;
+  double force_cap;
+  const double* t;            /* L x L real, one-body kinetic matrix of the energy */
+  const double* chol;         /* L x L x G real, column-major per vector */
+  const double* bg;           /* G real current background (NULL = zeros) */
+  const double* phiT_up;      /* L x Nup complex */
+  const double* phiT_dn;      /* L x Ndn complex */
+  double phiT_logw[2];
+  const double* half_rot_up;  /* Nup x L x G complex: phiT_up^H L_g, column-major (wfUpDaggerCholeskyVecs) */
+  const double* half_rot_dn;  /* Ndn x L x G complex */
+  const double* const_force;  /* G complex, used when force_type == 1 (NULL = zeros) */
+  int max_walkers;
+  int taylor_fixed;           /* 0 dynamicOrder, 1 fixedOrder */
+  int taylor_order;           /* fixedOrder: terms (0 = found on the first application, then frozen) */
+  int taylor_base;            /* baseTaylorOrder (<= 0: 3) */
+  double taylor_accuracy;     /* <= 0: 1e-10 */
+} afqmc_ph_desc;
+
+int afqmc_ph_create(afqmc_ctx ctx, const afqmc_ph_desc* desc, afqmc_ph* out);
+int afqmc_ph_destroy(afqmc_ph ph);
+/* one-body operators of the driver: which = 0 exp(-dt Kp), 1 exp(-dt/2 Kp), 2 exp(+dt/2 Kp).
+ * is_log = 0: Hop2is matrix B applied as B W (hop2isSD2isOperation.cpp:14-19);
+ * is_log = 1: LogHop2is matrix A applied as the Taylor series of exp(A) W (logHop2isSD2isOperation.cpp:99-195).
+ * matrix L x L complex, logw 1 complex */
+int afqmc_ph_set_onebody(afqmc_ph ph, int which, const double* matrix, const double* logw, int is_log);
+/* RealMaterialMolecule::updateBackGround (realMaterialMolecule.cpp:103-116): G real */
+int afqmc_ph_set_background(afqmc_ph ph, const double* bg);
+
+/* std::vector<SD2is> of the driver (tutorials/afqmcPhaseless/include/afqmcPhaseless.h:29): wf nw x (L x max(Nup,Ndn)) complex */
+int afqmc_ph_walkers_create(afqmc_ph ph, int nw, afqmc_ph_walkers* out);
+int afqmc_ph_walkers_destroy(afqmc_ph_walkers w);
+int afqmc_ph_walkers_upload(afqmc_ph_walkers w, const double* wf, const double* logw, const int* alive);
+int afqmc_ph_walkers_download(afqmc_ph_walkers w, double* wf, double* logw, int* alive);
+
+/* random numbers: as afqmc_cp_set_seed / afqmc_cp_set_uniforms; a phaseless step takes 2G uniforms
+ * per alive walker (u1, u2 of gaussianHao, random_hao.cpp:78-85) */
+int afqmc_ph_set_seed(afqmc_ph ph, uint64_t seed, uint64_t counter);
+int afqmc_ph_get_counter(afqmc_ph ph, uint64_t* counter, uint64_t* walker_steps);
+int afqmc_ph_set_uniforms(afqmc_ph ph, const double* u, int64_t n);
+int afqmc_ph_skip_uniforms(afqmc_ph ph, uint64_t n);
+int afqmc_ph_set_ET(afqmc_ph ph, double ET);
+int afqmc_ph_get_ET(afqmc_ph ph, double* ET);
+
+/* SD2sSD2isOperation::returnLogOverlap (SD2sSD2isOperation.cpp:84-94): logO nw complex */
+int afqmc_ph_log_overlap(afqmc_ph ph, afqmc_ph_walkers w, double* logO);
+/* RealMaterialMoleculeMeasureFixedSD2sSD2is::calculateCholeskyBg (:237-256): bg nw x G complex */
+int afqmc_ph_background(afqmc_ph ph, afqmc_ph_walkers w, double* bg);
+/* CholeskyReal::getTwoBodySampleFromAuxForce (choleskyReal.cpp:104-124) followed by
+ * LogHop2isSD2isOperation::applyToRight: aux, force nw x G complex; sample_logw nw complex out (may be NULL);
+ * max_order out (may be NULL); the per-walker orders by afqmc_ph_taylor_orders */
+int afqmc_ph_sample_apply(afqmc_ph ph, afqmc_ph_walkers w, const double* aux, const double* force,
+                          double* sample_logw, int* max_order);
+/* the last sample matrices sum_g sqrt(-dt) x_g L_g: nw x L x L complex */
+int afqmc_ph_sample_matrix(afqmc_ph ph, afqmc_ph_walkers w, double* M);
+int afqmc_ph_taylor_orders(afqmc_ph ph, afqmc_ph_walkers w, int* orders);
+/* Hop2isSD2isOperation / LogHop2isSD2isOperation applyToRight with one-body operator `which` */
+int afqmc_ph_onebody(afqmc_ph ph, afqmc_ph_walkers w, int which);
+/* SD2is::stabilize (SD2is.cpp:48-61) by CholeskyQR2 */
+int afqmc_ph_stabilize(afqmc_ph ph, afqmc_ph_walkers w);
+/* projectExpMinusDtKExpMinusDtV (afqmcPhaselessProjection.cpp:75-121): force and aux records nw x G complex
+ * (may be NULL) */
+int afqmc_ph_step(afqmc_ph ph, afqmc_ph_walkers w, double* force_out, double* aux_out);
+/* popControl (afqmcPhaselessProjection.cpp:150-202) */
+int afqmc_ph_pop_control(afqmc_ph ph, afqmc_ph_walkers w, double* weights_out, int* table_out, double* stats);
+/* addMixedMeasurement (afqmcPhaselessMeasure.cpp:9-26) */
+int afqmc_ph_measure(afqmc_ph ph, afqmc_ph_walkers w);
+/* accumulated den, HNum, TNum (3 complex); choleskyBgNum and choleskyExNum (G complex each, may be NULL) */
+int afqmc_ph_energy(afqmc_ph ph, double* out, double* bg_num, double* ex_num);
+int afqmc_ph_reset_measure(afqmc_ph ph);
+/* ET = Re(HNum/den), then reset (the ET half of adjustETAndBackGroundThenResetMeasurement,
+ * afqmcPhaselessMeasure.cpp:36-45; the background half is afqmc_ph_set_background) */
+int afqmc_ph_adjust_ET(afqmc_ph ph);
+
 /* ------------------------------------------------------------------ testing hooks
  * One kernel family on host data, for unit tests of the device building blocks.
  * No reference counterpart: tensor_2d_bl_cpu.cpp's gmm / LUconstruct+inverse /
  * QR are what they are checked against. */
-/* C_b = op(A_b) diag(scale) B_b; A M x K (op N) or K x M (op T/C), strideA = 0 shares A */
+/* C_b = op(A_b) diag(scale) B_b; A M x K (op N) or K x M (op T/C), strideA = 0 shares A;
+ * opa = 3: op N with A real (A then holds real doubles) */
 int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K, int batch, const double* A,
                      long long strideA, const double* B, const double* scale, int scale_group, double* C);
 /* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting) */

@@ -41,8 +41,10 @@ def zgemm(opa, A, B, precision=0, batch=1, strideA=0, scale=None, scale_group=1)
     else:
         A3 = A
         strideA = A.shape[1] * A.shape[2]
-    M = A3.shape[1] if opa == 0 else A3.shape[2]
+    M = A3.shape[1] if opa in (0, 3) else A3.shape[2]
     Abuf = np.ascontiguousarray(np.transpose(A3, (0, 2, 1)))  # column-major per matrix
+    if opa == 3:  # real A
+        Abuf = np.ascontiguousarray(Abuf.real)
     Bbuf = np.ascontiguousarray(np.transpose(B, (0, 2, 1)))
     Cbuf = np.zeros((batch, N, M), dtype=np.complex128)
     sbuf = None if scale is None else np.ascontiguousarray(scale, dtype=np.complex128)

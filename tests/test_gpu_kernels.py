@@ -27,6 +27,18 @@ def test_zgemm_matches_numpy(precision, opa, shape):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("shape", [(37, 45, 29), (130, 70, 97), (400, 8, 30)])
+def test_zgemm_real_a_matches_numpy(precision, shape):
+    """op N with a real A (the Cholesky-vector sample matrix): only Re(A) is read."""
+    rng = np.random.default_rng(sum(shape) + precision)
+    M, N, K = shape
+    A = rng.uniform(-1, 1, (M, K))
+    B = crand(rng, K, N)
+    C = zgemm(3, A + 0j, B, precision)[0]
+    assert np.max(np.abs(C - A @ B)) < TOL[precision] * np.sqrt(K) * 4
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_zgemm_batched_strided_and_scaled(precision):
     rng = np.random.default_rng(3)
     batch, M, N, K = 5, 33, 20, 40
