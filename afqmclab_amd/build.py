@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libafqmc_mi355x.so")
+HOST = os.path.join(HERE, "host")
+CP_RUN = os.path.join(HERE, "afqmc_cp_run")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AFQMC_OFFLOAD_ARCH", "gfx950")
@@ -55,8 +57,16 @@ def build(verbose=True, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ host driver on the C-ABI (plain g++: host code only, linked against the library)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HOST, "cp_run.cpp"), "-o", CP_RUN + ".tmp", "-L", HERE, "-lafqmc_mi355x",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host driver build failed:\n{r.stderr}")
+    os.replace(CP_RUN + ".tmp", CP_RUN)
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources and {CP_RUN}")
     return LIB
 
 

@@ -200,6 +200,209 @@ __global__ __launch_bounds__(TG * TG) void gj_kernel(SmallDenseArgs p) {
   }
 }
 
+// Packed pivot key of a candidate row: the bits of the non-negative score |re|+|im| (monotone as
+// an unsigned integer) with the low 8 bits replaced by 255 - row, and bit 63 set.  The maximum
+// key is the largest score, ties (to 2^-44 relative) going to the smallest row, as izamax does.
+__device__ __forceinline__ unsigned long long pivot_key(zd v, int row) {
+  double s = abs1(v);
+  if (!(s >= 0.0)) s = 0.0;  // NaN input: still pick a valid unused row (reported singular)
+  return ((unsigned long long)__double_as_longlong(s) & ~0xFFull) | (unsigned long long)(255 - row) | (1ull << 63);
+}
+
+// Gauss-Jordan with look-ahead: the threads that own column k+1 (TG consecutive lanes of one
+// wave) update that column first during step k, find its pivot with a reduction inside their
+// lane group and publish the column and the pivot row through LDS, then do the rest of their
+// update.  The pivot search is thereby off the critical path of the other waves, and each
+// step has one barrier.  Same pivots, logdet and inverse as the sweep above.
+template <typename T, int TG, int E>
+__global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
+  const int bidx = blockIdx.x;
+  if (p.alive && !p.alive[bidx / p.alive_div]) return;
+  const int N = p.N;
+  const int tid = threadIdx.x;
+  const int tr = tid % TG, tc = tid / TG;
+  typedef cplx<T> ct;
+  const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
+  ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
+
+  __shared__ zd colbuf[2][NMAX];
+  __shared__ int pivrow[2];
+  __shared__ zd pivs[NMAX];
+  __shared__ int perm[NMAX];
+  __shared__ int qinv[NMAX];
+  __shared__ unsigned char seen[NMAX];
+
+  zd x[E][E];
+#pragma unroll
+  for (int a = 0; a < E; ++a)
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int i = tr + TG * a, j = tc + TG * c;
+      x[a][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+    }
+  unsigned used = 0u;  // bit a: row tr + TG*a has been a pivot row
+  int singular = 0;
+
+  // the owners of column `col` publish it and its pivot row (reduction over their TG lanes)
+  auto publish = [&](int col, int buf) {
+    const int cc = col / TG;
+    unsigned long long key = 0ull;
+#pragma unroll
+    for (int a = 0; a < E; ++a) {
+      const int i = tr + TG * a;
+      zd v = cmk<double>(0, 0);
+#pragma unroll
+      for (int c = 0; c < E; ++c)
+        if (c == cc) v = x[a][c];
+      if (i < N) {
+        colbuf[buf][i] = v;
+        if (!((used >> a) & 1u)) {
+          const unsigned long long kk = pivot_key(v, i);
+          key = kk > key ? kk : key;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = TG / 2; o > 0; o >>= 1) {
+      const unsigned long long ok = (unsigned long long)__shfl_xor((long long)key, o, 64);
+      key = ok > key ? ok : key;
+    }
+    if (tr == 0) {
+      // bit 63 marks a candidate: at step col there are N - col >= 1 unused rows
+      const int bi = 255 - (int)(key & 0xFFull);
+      pivrow[buf] = (bi >= 0 && bi < N) ? bi : 0;
+    }
+  };
+
+  if (tc == 0) publish(0, 0);
+  __syncthreads();
+
+  for (int k = 0; k < N; ++k) {
+    const int buf = k & 1;
+    const int pr = pivrow[buf];
+    const zd piv = colbuf[buf][pr];
+    zd inv = cmk<double>(0, 0);
+    if (abs1(piv) > 0.0) {
+      inv = cinv(piv);
+    } else {
+      singular = 1;
+    }
+    if (tid == 0) {
+      perm[k] = pr;
+      pivs[k] = piv;
+    }
+    const bool prow = (tr == pr % TG);
+    const int ar = pr / TG;
+    if (prow) used |= 1u << ar;
+    zd rowv[E];
+    {
+      const int src = (pr % TG) + TG * (tc % (64 / TG));
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        zd v = cmk<double>(0, 0);
+#pragma unroll
+        for (int a = 0; a < E; ++a)
+          if (a == ar) v = x[a][c];
+        v.re = __shfl(v.re, src, 64);
+        v.im = __shfl(v.im, src, 64);
+        const int j = tc + TG * c;
+        rowv[c] = (j == k) ? cmk<double>(0, 0) : cmul(v, inv);
+      }
+    }
+    zd cv[E];
+#pragma unroll
+    for (int a = 0; a < E; ++a) {
+      const int i = tr + TG * a;
+      cv[a] = i < N ? colbuf[buf][i] : cmk<double>(0, 0);
+    }
+    // look-ahead: column k+1 first (its owners), then publish it
+    const bool ahead = (k + 1 < N) && (tc == (k + 1) % TG);
+    const int c1 = (k + 1) / TG;
+    if (ahead) {
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        if (c != c1) continue;
+#pragma unroll
+        for (int a = 0; a < E; ++a) x[a][c] = csub(x[a][c], cmul(cv[a], rowv[c]));
+        if (prow) {
+#pragma unroll
+          for (int a = 0; a < E; ++a)
+            if (a == ar) x[a][c] = cmk<double>(-rowv[c].re, -rowv[c].im);
+        }
+      }
+      publish(k + 1, buf ^ 1);
+    }
+    // the rest of the rank-1 update, then the pivot row and column k
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      if (ahead && c == c1) continue;
+#pragma unroll
+      for (int a = 0; a < E; ++a) x[a][c] = csub(x[a][c], cmul(cv[a], rowv[c]));
+    }
+    if (prow) {
+#pragma unroll
+      for (int a = 0; a < E; ++a)
+        if (a == ar)
+#pragma unroll
+          for (int c = 0; c < E; ++c) x[a][c] = cmk<double>(-rowv[c].re, -rowv[c].im);
+    }
+    if (tc == k % TG) {
+      const int kc = k / TG;
+#pragma unroll
+      for (int c = 0; c < E; ++c)
+        if (c == kc)
+#pragma unroll
+          for (int a = 0; a < E; ++a) x[a][c] = (tr + TG * a == pr) ? inv : cmul(cv[a], inv);
+    }
+    __syncthreads();
+  }
+  if (tid < N) qinv[perm[tid]] = tid;
+  if (tid == 0) {
+    double lognorm = 0.0;
+    zd phase = cmk<double>(1.0, 0.0);
+    if (!singular) {
+      for (int k = 0; k < N; ++k) {
+        const zd pv = pivs[k];
+        const double ap = cabs(pv);
+        lognorm += log(ap);
+        phase = cmul(phase, cmk<double>(pv.re / ap, pv.im / ap));
+      }
+    }
+    int visited_parity = 0;
+    for (int i = 0; i < N; ++i) seen[i] = 0;
+    for (int i = 0; i < N; ++i) {
+      if (seen[i]) continue;
+      int len = 0, j = i;
+      while (len < N && !seen[j]) { seen[j] = 1; j = perm[j]; ++len; }
+      visited_parity ^= (len - 1) & 1;
+    }
+    zd ph = visited_parity ? cmk<double>(-phase.re, -phase.im) : phase;
+    zd ld;
+    if (singular) {
+      ld = cmk<double>(-1e300, 0.0);
+    } else {
+      ld = cmk<double>(lognorm + 0.0, atan2(ph.im, ph.re));
+      ld.re += log(hypot(ph.re, ph.im));
+    }
+    p.logdet[bidx] = ld;
+    if (p.info) p.info[bidx] = singular;
+  }
+  if (!p.write_inverse) return;
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < E; ++a) {
+    int i = tr + TG * a;
+    if (i >= N) continue;
+    int row = qinv[i];
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int j = tc + TG * c;
+      if (j >= N) continue;
+      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(x[a][c]);
+    }
+  }
+}
+
 template <typename T, int TG, int E>
 __global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
   const int bidx = blockIdx.x;

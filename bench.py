@@ -119,6 +119,45 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched):
     return dict(elapsed=elapsed, walker_steps=wsteps, timing=timing, energy=E, den=e["den"])
 
 
+def run_phaseless(ctx, mol, precision, nw, steps, warmup, seed, meas=5):
+    """Phaseless walker path on the synthetic real-materials Hamiltonian: one step is
+    projectExpMinusDtKExpMinusDtV of all walkers, QR every 10, comb every 10, mixed estimator
+    (with the G * Ls * N^2 exchange) every `meas` steps."""
+    from afqmclab_amd.phaseless import AfqmcPhaseless, PhaselessMethod
+    m = PhaselessMethod(dt=0.005, walkerSizePerThread=nw, mgsStep=10, popControlStep=10, ET=-10.0, seed=seed)
+    drv = AfqmcPhaseless(ctx, mol, m, precision=precision)
+    drv.initialWalker()
+    drv.estimateMemory()
+    drv.prop.onebody(drv.walker, 1)
+    idx = [0, 0]
+
+    def one_step(i):
+        if i % meas == 0:
+            drv.prop.measure(drv.walker)
+        drv.projectOneStep(idx)
+
+    for i in range(warmup):
+        one_step(i)
+    ctx.synchronize()
+    drv.prop.reset_measure()
+    ws0 = drv.prop.walker_steps()
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        one_step(i)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    timing = ctx.timing()
+    ctx.set_timing(False)
+    wsteps = drv.prop.walker_steps() - ws0
+    e = drv.prop.energy()
+    E = (e["HNum"] / e["den"]) if abs(e["den"]) > 0 else complex("nan")
+    drv.walker.close()
+    drv.prop.close()
+    return dict(elapsed=elapsed, walker_steps=wsteps, timing=timing, energy=E)
+
+
 def cpu_baseline(model, phi, nw_cpu, budget_s, sched, seed):
     """Oracle (dense reference representation) on a bounded sample of the same workload."""
     from oracle.blocks import SD
@@ -158,6 +197,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-walkers", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ph-steps", type=int, default=60, help="phaseless leg steps (0 = skip)")
+    ap.add_argument("--ph-walkers", type=int, default=1024)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -227,6 +268,30 @@ def main():
         f, mx = results["fp64"], results["mixed"]
         out["mixed_vs_fp64"] = (mx["walker_steps"] / mx["elapsed"]) / (f["walker_steps"] / f["elapsed"])
         out["dE_per_site_mixed_vs_fp64"] = abs((mx["energy"] - f["energy"]) / Ls)
+    if args.ph_steps > 0:
+        from afqmclab_amd.phaseless import synthetic_molecule
+        t0 = time.time()
+        mol = synthetic_molecule(L=200, Nup=40, Ndn=40, G=1600, seed=12345)
+        log(f"[bench] phaseless Hamiltonian L=200 G=1600: {time.time() - t0:.1f}s")
+        ph = {}
+        for path in args.paths.split(","):
+            precision = 1 if path == "mixed" else 0
+            log(f"[bench] phaseless {path}: {args.ph_walkers} walkers, 5 warmup + {args.ph_steps} steps")
+            rr = run_phaseless(ctx, mol, precision, args.ph_walkers, args.ph_steps, 5, args.seed)
+            log(f"[bench]   phaseless {path}: {rr['walker_steps'] / rr['elapsed']:.1f} walker-steps/s, "
+                f"E={rr['energy']:.8f}")
+            ts, ns = rr["timing"]["ph_sample"]
+            ph[path] = {
+                "walker_steps_per_s": rr["walker_steps"] / rr["elapsed"],
+                "ms_per_step": rr["elapsed"] / args.ph_steps * 1e3,
+                "energy": [rr["energy"].real, rr["energy"].imag],
+                "kernel_ms": {k: round(v[0], 3) for k, v in rr["timing"].items() if v[1]},
+            }
+        out["phaseless"] = {"workload": "phaseless AFQMC, synthetic molecule L=200 Nup=Ndn=40 G=1600 RHF trial "
+                                        f"dt=0.005, {args.ph_walkers} walkers, QR/comb every 10, measure every 5",
+                            "steps": args.ph_steps, "paths": ph}
+        if "fp64" in ph and "mixed" in ph:
+            out["phaseless"]["mixed_vs_fp64"] = ph["mixed"]["walker_steps_per_s"] / ph["fp64"]["walker_steps_per_s"]
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle, {args.cpu_walkers} walkers, ~{args.cpu_seconds}s")
         ws_s, nst = cpu_baseline(model, phi, args.cpu_walkers, args.cpu_seconds, sched, args.seed)

@@ -51,3 +51,16 @@ def test_errors_are_reported_not_fatal():
     rc = lib.afqmc_cp_create(None, None, None)
     assert rc < 0
     assert b"null" in lib.afqmc_last_error()
+
+
+EXE = os.path.join(ROOT, "afqmclab_amd", "afqmc_cp_run")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="host driver not built (run __graft_entry__.build())")
+def test_host_driver_reports_missing_input(tmp_path):
+    """The C++ host driver reads afqmc_param before touching the device; a missing file is an
+    error message and exit code 1, not a crash."""
+    import subprocess
+    r = subprocess.run([EXE, str(tmp_path / "nothing")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "cannot open" in r.stderr

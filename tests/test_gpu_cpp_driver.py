@@ -1,0 +1,73 @@
+"""The C++ host driver (afqmclab_amd/host, built as afqmclab_amd/afqmc_cp_run) runs the
+constraint-path loop through the C-ABI: same energies as the Python driver on the same
+library calls, and the oracle's per-block HNum/den within 1e-9."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from afqmclab_amd.cp import AfqmcConstraintPath, CPMethod  # noqa: E402
+from afqmclab_amd.model import HubbardModel, free_electron_trial, spin_block_trial  # noqa: E402
+from oracle.blocks import SD  # noqa: E402
+from oracle.drivers import AfqmcConstraintPath as OracleCP, CPMethod as OracleMethod  # noqa: E402
+from oracle.models import HubbardSOC  # noqa: E402
+from oracle.rng import UniformStream  # noqa: E402
+from tests.gpu_util import ctx  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "afqmclab_amd", "afqmc_cp_run")
+
+
+def write_inputs(d, model, phi, m, nup, spin_blocks):
+    with open(os.path.join(d, "afqmc_param"), "w") as f:
+        for k, v in m.items():
+            f.write(f"{k:36s}{v}\n")
+    with open(os.path.join(d, "model_dims"), "w") as f:
+        f.write(f"{model.L} {model.N} {nup} {int(spin_blocks)}\n")
+
+    def put(name, a, dtype):
+        np.asfortranarray(np.asarray(a, dtype=dtype)).ravel(order="F").tofile(os.path.join(d, name + ".bin"))
+
+    put("K", model.K, np.complex128)
+    put("expMinusDtK", model.exp_minus_alpha_k(m["dt"]), np.complex128)
+    put("expMinusHalfDtK", model.exp_minus_alpha_k(0.5 * m["dt"]), np.complex128)
+    put("expHalfDtK", model.exp_minus_alpha_k(-0.5 * m["dt"]), np.complex128)
+    put("phiT", phi, np.complex128)
+    for name in ("U", "mu", "hx", "hy", "hz"):
+        put(name, getattr(model, name), np.float64)
+
+
+def read_dat(path):
+    v = np.loadtxt(path, ndmin=2)
+    return v[:, 0] + 1j * v[:, 1]
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_cpp_driver_matches_python_driver_and_oracle(tmp_path, mode):
+    nup = ndn = 5
+    model = HubbardModel.square(4, 4, nup + ndn, 4.0)
+    phi = free_electron_trial(model) if mode == "dense" else spin_block_trial(model, nup, ndn)
+    sb = mode == "blocks"
+    m = dict(dt=0.01, thermalSize=20, writeNumber=3, measureNumberPerWrite=2, measureSkipStep=5,
+             walkerSizePerThread=16, decompType="densitySpin", forceType="dynamicForce", forceCap=1.5,
+             mgsStep=10, popControlStep=10, ET=-10.0, ETAdjustStep=10, ETAdjustMaxSize=20, seed=31)
+    write_inputs(str(tmp_path), model, phi, m, nup, sb)
+    print(f"[cpp driver] mode={mode}", flush=True)
+    r = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    H = read_dat(tmp_path / "HNum.dat")
+    D = read_dat(tmp_path / "den.dat")
+    kw = dict(nup=nup, spin_blocks=True) if sb else dict(spin_blocks=False)
+    dev = AfqmcConstraintPath(ctx(), model, CPMethod(**m), phi, **kw)
+    dev.run()
+    assert len(H) == len(dev.energies) == m["writeNumber"]
+    for h, d, e in zip(H, D, dev.energies):
+        assert abs(h - e[0]) <= 1e-12 * abs(e[0]) and abs(d - e[1]) <= 1e-12 * abs(e[1])
+    om = HubbardSOC(model.L, model.N, model.K, U=model.U)
+    ref = OracleCP(OracleMethod(**m), om, UniformStream(31), phiT=SD(wf=phi))
+    ref.run()
+    for h, d, e in zip(H, D, ref.energies):
+        assert abs(h / d - e[0] / e[1]) < 1e-9 * abs(e[0] / e[1])

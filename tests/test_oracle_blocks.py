@@ -2,7 +2,9 @@
 closed forms / brute force.  CPU only."""
 import cmath
 import itertools
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -14,9 +16,12 @@ from oracle.blocks import (SD, SD2is, Hop, LogHop2is, HopSDOperation, LogHop2isS
 from oracle.rng import UniformStream, gaussians_from_uniforms, uniforms, seed_mix
 
 
+KATS = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_kats.json")))
+
+
 def col_major(vals, rows, cols):
     """Literal initialiser lists of the reference fill TensorHao in storage (column-major) order."""
-    a = np.array([complex(*v) if isinstance(v, tuple) else v for v in vals], dtype=np.complex128)
+    a = np.array([complex(*v) if isinstance(v, (tuple, list)) else v for v in vals], dtype=np.complex128)
     return a.reshape(rows, cols, order="F")
 
 
@@ -33,15 +38,12 @@ def test_solve_cosh_and_cos_identities():
 
 
 def test_exp2by2_literals():
-    # simple_fun_test.cpp:36-64
-    a, b, c = mathfun.exp2by2Matrix(1.2, -2.0, complex(3.0, -2.0))
-    assert abs(a - 24.33957375963294) < 1e-12
-    assert abs(b - 10.299809224500716) < 1e-12
-    assert abs(c - complex(13.162279251686476, -8.774852834457649)) < 1e-12
-    a, b, c = mathfun.exp2by2Matrix(0.0, 0.0, complex(3.0, 2.0))
-    assert a == pytest.approx(18.414569374170224, rel=4e-16)
-    assert b == pytest.approx(18.414569374170224, rel=4e-16)
-    assert abs(c - complex(15.299239013566734, 10.199492675711154)) < 1e-12
+    kat = KATS["exp2by2Matrix"]
+    for c in kat["cases"]:
+        a, b, z = mathfun.exp2by2Matrix(c["a"], c["b"], complex(*c["c"]))
+        assert abs(a - c["expect_a"]) < c["tol"]
+        assert abs(b - c["expect_b"]) < c["tol"]
+        assert abs(z - complex(*c["expect_c"])) < c["tol"]
     a, b, c = mathfun.exp2by2Matrix(1.0, 2.0, 0j)
     assert a == math.exp(1.0) and b == math.exp(2.0) and c == 0
 
@@ -58,26 +60,24 @@ def test_exp2by2_matches_matrix_exponential():
 
 
 # ---------------------------------------------------------------- walker QR
-WF43 = [(1.0, 1.0), (2.0, 1.0), (3.0, 1.0), (4.0, 1.0),
-        (2.0, 2.0), (3.0, 1.0), (5.0, 2.0), (6.0, 1.0),
-        (4.0, 1.0), (3.0, 3.0), (2.0, 2.0), (2.0, 6.0)]
+WF43 = KATS["walker_4x3"]
 
 
 def test_sd_normalize_literal():
-    # SDTest.cpp:158-173
+    kat = KATS["SD_normalize"]
     sd = SD(wf=col_major(WF43, 4, 3))
     logw, ratio = sd.normalize_ratio()
-    assert logw == pytest.approx(complex(3.35769169316734, 0.0), rel=1e-14)
-    assert ratio == pytest.approx(0.19286583895005868, rel=1e-14)
+    assert logw == pytest.approx(complex(kat["logw"], 0.0), rel=1e-14)
+    assert ratio == pytest.approx(kat["ratio"], rel=1e-14)
     assert np.allclose(sd.wf.conj().T @ sd.wf, np.eye(3), atol=1e-14)
 
 
 def test_sd2is_normalize_literal():
-    # SD2isTest.cpp:164-179
-    w = SD2is(col_major(WF43, 4, 3), 3, 2)
+    kat = KATS["SD2is_normalize"]
+    w = SD2is(col_major(WF43, 4, 3), kat["Nup"], kat["Ndn"])
     logw, ratio = w.normalize_ratio()
-    assert logw == pytest.approx(complex(5.2382917510141205, 0.0), rel=1e-14)
-    assert ratio == pytest.approx(0.19286583895005868, rel=1e-14)
+    assert logw == pytest.approx(complex(kat["logw"], 0.0), rel=1e-14)
+    assert ratio == pytest.approx(kat["ratio"], rel=1e-14)
 
 
 def test_qr_sign_rule_and_invariants():
@@ -93,32 +93,25 @@ def test_qr_sign_rule_and_invariants():
 
 
 # ---------------------------------------------------------------- Taylor exp
-TAYLOR_A = [(2, 3), (4, 2), (1, 1), (1, -1), (0, 0.3), (1, -1), (3, 2), (2, 3), (0, 0.5)]
-TAYLOR_W = [(1, 0), (2, 0), (3, 0), (3, 0), (4, 0), (5, 0)]
-RIGHT = [(0.8717255743431891, -0.06695756061260984), (1.9019539751922434, -0.11272541520548454),
-         (2.9713563453236675, -0.0038676809054978764), (2.753351116138562, -0.14393925193667553),
-         (3.7836486105153155, -0.21511811236274694), (4.932649990968739, -0.012861814634312912)]
-LEFT = [(0.8708226627963166, 0.09656591603485742), (1.961995737773784, -0.034019035572233965),
-        (2.9316294741182647, 0.09182786294989435), (2.7315235332357157, 0.21256279549696155),
-        (3.924363675467334, -0.0680085413131974), (4.833174488748831, 0.19818901143889675)]
+TAYLOR = KATS["LogHop2isSD2isOperation"]
 
 
-@pytest.mark.parametrize("flag,order,acc,expect_order,tol", [
-    ("fixedOrder", 8, 1e-8, 8, 1e-12), ("fixedOrder", 0, 1e-8, 6, 1e-8), ("dynamicOrder", 0, 1e-8, 6, 1e-8)])
-def test_taylor_literals(flag, order, acc, expect_order, tol):
-    # logHop2isSD2isOperationTest.cpp:25-94 (L=3, Nup=1, Ndn=2)
-    A = -0.01 * col_major(TAYLOR_A, 3, 3)
-    op = LogHop2is(A, complex(1.2, 1.5))
-    w = SD2is(col_major(TAYLOR_W, 3, 2), 1, 2, 1.6)
-    t = LogHop2isSD2isOperation(flag, order, acc)
+@pytest.mark.parametrize("case", TAYLOR["cases"], ids=lambda c: f'{c["flag"]}-{c["order"]}')
+def test_taylor_literals(case):
+    k = TAYLOR
+    L, nup, ndn = k["L"], k["Nup"], k["Ndn"]
+    A = k["A_scale"] * col_major(k["A"], L, L)
+    op = LogHop2is(A, complex(*k["A_logw"]))
+    w = SD2is(col_major(k["W"], L, max(nup, ndn)), nup, ndn, complex(*k["W_logw"]))
+    t = LogHop2isSD2isOperation(case["flag"], case["order"], case["accuracy"])
     r = t.applyToRight(op, w)
-    assert t.currentOrder == expect_order
-    assert np.max(np.abs(r.wf - col_major(RIGHT, 3, 2))) < tol
-    assert r.logw == complex(2.8, 1.5)
+    assert t.currentOrder == case["expect_order"]
+    assert np.max(np.abs(r.wf - col_major(k["right"], L, 2))) < case["tol"]
+    assert r.logw == complex(*k["right_logw"])
     l = t.applyToLeft(op, w)
-    assert t.currentOrder == expect_order
-    assert np.max(np.abs(l.wf - col_major(LEFT, 3, 2))) < tol
-    assert l.logw == complex(2.8, -1.5)
+    assert t.currentOrder == case["expect_order"]
+    assert np.max(np.abs(l.wf - col_major(k["left"], L, 2))) < case["tol"]
+    assert l.logw == complex(*k["left_logw"])
 
 
 # ---------------------------------------------------------------- one-body / HS apply

@@ -1,6 +1,8 @@
 """Pin the oracle's models, comb and ED against reference literals, invariants
 and brute force.  CPU only."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -13,15 +15,13 @@ from oracle.models import (HubbardSOC, HubbardSOCMeasureFixedSDSD, RealMaterialM
                            RealMaterialMoleculeMeasureFixedSD2sSD2is, square_lattice_K, synthetic_molecule)
 from oracle.rng import UniformStream
 
+KATS = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_kats.json")))
+
 
 # ---------------------------------------------------------------- comb
 def test_popcheck_literals():
-    # populationControlTest.cpp:12-27
-    a = [i + 1.0 for i in range(10)]
-    assert comb.popCheck(a) == (1.0, 10.0, 5.5)
-    a = [1.0] * 10
-    a[5] = 6.0
-    assert comb.popCheck(a) == (1.0, 6.0, 1.5)
+    for c in KATS["popCheck"]["cases"]:
+        assert comb.popCheck([float(x) for x in c["weights"]]) == tuple(c["expect"])
 
 
 def test_popconfiguration_invariants():
@@ -204,8 +204,8 @@ def test_phaseless_sample_logw_and_kp():
 
 # ---------------------------------------------------------------- ED
 def twisted_ring():
-    # realMaterialTest.cpp:242-275
-    a = complex(-0.309016994374947, 0.951056516295154)
+    k = KATS["twisted_ring"]
+    a = complex(*k["a"])
     K = np.zeros((4, 4), dtype=complex)
     K[0, 3] = a; K[0, 1] = a.conjugate(); K[1, 2] = a.conjugate(); K[1, 0] = a
     K[2, 3] = a.conjugate(); K[2, 1] = a; K[3, 0] = a.conjugate(); K[3, 2] = a
@@ -213,12 +213,13 @@ def twisted_ring():
 
 
 def test_lanczos_twisted_ring_literals():
+    k = KATS["twisted_ring"]
     K = twisted_ring()
-    ed = HubbardED(K, K, np.full(4, 4.0), 2, 3)
+    ed = HubbardED(K, K, np.full(k["L"], k["U"]), k["Nup"], k["Ndn"])
     E, Kx, Vx, _ = ed.ground_state()
-    assert abs(E - 0.877119735524673) < 1e-10
-    assert abs(Kx - (-3.87084415356266)) < 1e-10
-    assert abs(Vx - 4.74796388908733) < 1e-10
+    assert abs(E - k["E"]) < k["tol"]
+    assert abs(Kx - k["K_energy"]) < k["tol"]
+    assert abs(Vx - k["V_energy"]) < k["tol"]
 
 
 def test_lanczos_u0_sum_of_eigenvalues():
