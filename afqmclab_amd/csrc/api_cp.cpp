@@ -80,7 +80,7 @@ struct afqmc_cp_s {
   DevBuf<double> ubuf;
   // workspace
   DevBuf<char> ovl[2], scale[2], d2x2, qrS[2], qr64[2], qr64b[2];
-  DevBuf<zd> ldet[2], qrld[2], logO, gpart[3][2], per_walker, site_acc, den;
+  DevBuf<zd> ldet[2], qrld[2], logO, gpart[3][2], per_walker, site_acc, site_part, den;
   DevBuf<int> info[2], qrinfo[2], prefix, table, comb_num;
   DevBuf<double> weight, comb_prob, force_rec;
   DevBuf<int> aux_rec;
@@ -392,7 +392,7 @@ int measure_impl(afqmc_cp cp, afqmc_walkers w) {
     e.ntile[b] = cp->ntile[b]; e.Mb[b] = cp->Mb[b];
   }
   e.U = cp->U.p; e.mu = cp->mu.p; e.hx = cp->hx.p; e.hy = cp->hy.p; e.hz = cp->hz.p;
-  e.alive = w->alive.p; e.den = cp->den.p; e.per_walker = cp->per_walker.p; e.site_acc = cp->site_acc.p; e.sc = cp->sc.p;
+  e.alive = w->alive.p; e.den = cp->den.p; e.per_walker = cp->per_walker.p; e.site_acc = cp->site_acc.p; e.site_part = cp->site_part.p; e.sc = cp->sc.p;
   return launch_energy(e, s);
 }
 
@@ -535,6 +535,7 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
   if (!rc) rc = cp->weight.alloc(nw);
   if (!rc) rc = cp->per_walker.alloc((size_t)nw * 8);
   if (!rc) rc = cp->site_acc.alloc((size_t)4 * L);
+  if (!rc) rc = cp->site_part.alloc((size_t)SITE_CHUNKS * 4 * L);
   if (!rc) rc = cp->force_rec.alloc((size_t)nw * L);
   if (!rc) rc = cp->aux_rec.alloc((size_t)nw * L);
   if (!rc) rc = cp->sc.alloc(1);

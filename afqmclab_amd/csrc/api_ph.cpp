@@ -365,17 +365,19 @@ int ph_measure_impl(afqmc_ph ph, afqmc_ph_walkers w) {
                         : launch_rowdot_sum<double>(w->nw, n, ph->X[sp].p, n, ph->wdtT[sp].p, ph->tkin.p, sp, s))))
       return rc;
   }
-  // exchange: D_s = A_s[g] theta_s stacked over g ((G Ns) x Ns per walker), in walker chunks
+  // exchange: D_s = A_s[g] theta_s stacked over g, (G Ns) x Ns per walker, for a chunk of walkers
+  // at once: one GEMM with the shared half-rotated vectors as A and the chunk's theta matrices
+  // side by side as B (N = Ns * chunk), A panels reused across the chunk (N tiles fastest)
   for (int sp = 0; sp < 2; ++sp) {
     const int Ns = ph->Ns[sp];
     for (int w0 = 0; w0 < w->nw; w0 += ph->dchunk) {
       const int nb = std::min(ph->dchunk, w->nw - w0);
       GemmArgs a = gbase();
-      a.M = ph->G * Ns; a.N = Ns; a.K = ph->L; a.batch = nb;
-      a.A = ph->Ahalf[sp].p; a.lda = ph->L; a.strideA = 0;
-      a.B = (char*)ph->X[sp].p + ph->esz() * (size_t)w0 * ph->L * Ns; a.ldb = ph->L; a.strideB = (long long)ph->L * Ns;
-      a.C = ph->Dbuf.p; a.ldc = (long long)ph->G * Ns; a.strideC = (long long)ph->G * Ns * Ns;
-      a.alive = w->alive.p + w0;
+      a.M = ph->G * Ns; a.N = Ns * nb; a.K = ph->L; a.batch = 1;
+      a.A = ph->Ahalf[sp].p; a.lda = ph->L;
+      a.B = (char*)ph->X[sp].p + ph->esz() * (size_t)w0 * ph->L * Ns; a.ldb = ph->L;
+      a.C = ph->Dbuf.p; a.ldc = (long long)ph->G * Ns;
+      a.n_fastest = 1;
       if ((rc = launch_cgemm<T>(OP_T, EPI_STORE, a, s))) return rc;
       if ((rc = launch_exchange_trace<T>(nb, ph->G, Ns, ph->Dbuf.p, ph->ex.p + (size_t)w0 * ph->G, sp, s))) return rc;
     }
@@ -500,9 +502,9 @@ int afqmc_ph_create(afqmc_ctx ctx, const afqmc_ph_desc* d, afqmc_ph* out) {
   if (!rc) rc = ph->xs.alloc_bytes(es * (size_t)G * nw);
   if (!rc) rc = ph->Msamp.alloc_bytes(es * (size_t)L2 * nw);
   for (int k = 0; k < 2 && !rc; ++k) rc = ph->Tk[k].alloc_bytes(es * wsz * nw);
-  // exchange chunk: keep the (G Ns x Ns) per-walker blocks under ~1 GiB
+  // exchange chunk: keep the (G Ns x Ns) per-walker blocks under 4 GiB
   const long long dper = (long long)G * ph->Nmax * ph->Nmax * (long long)es;
-  ph->dchunk = (int)std::max<long long>(1, std::min<long long>(nw, (1ll << 30) / std::max<long long>(dper, 1)));
+  ph->dchunk = (int)std::max<long long>(1, std::min<long long>(nw, (4ll << 30) / std::max<long long>(dper, 1)));
   if (!rc) rc = ph->Dbuf.alloc_bytes((size_t)dper * ph->dchunk);
   if (!rc) rc = ph->qrS.alloc_bytes(sizeof(zd) * (size_t)ph->Nmax * ph->Nmax * nw);
   if (!rc) rc = ph->qrld.alloc(nw);

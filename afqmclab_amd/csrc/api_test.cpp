@@ -10,6 +10,12 @@
 
 namespace {
 
+// plain streaming copy: the reference point for HBM traffic counters
+__global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 template <typename T>
 std::vector<cplx<T>> narrow(const double* p, size_t n) {
   std::vector<cplx<T>> v(n);
@@ -115,6 +121,34 @@ int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const 
   ScopedDevice sd(ctx);
   return precision ? gj_t<float>(ctx, 0, N, batch, A, Ainv, logdet, info)
                    : gj_t<double>(ctx, 0, N, batch, A, Ainv, logdet, info);
+}
+
+int afqmc_test_copy(afqmc_ctx ctx, long long bytes, int reps, double* ms) {
+  REQUIRE(ctx && bytes > 0 && bytes % 16 == 0 && reps > 0, "afqmc_test_copy: bytes must be a positive multiple of 16");
+  ScopedDevice sd(ctx);
+  hipStream_t s = ctx->stream;
+  DevBuf<float4> a, b;
+  const long long n = bytes / 16;
+  int rc = a.alloc((size_t)n);
+  if (!rc) rc = b.alloc((size_t)n);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(a.p, 0, (size_t)bytes, s));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) {
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, a.p, b.p, n);
+    HIP_LAUNCH_CHECK();
+  }
+  HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return 0;
 }
 
 int afqmc_test_chol_inverse(afqmc_ctx ctx, int N, int batch, const double* S, double* Rinv, double* logsum,

@@ -408,12 +408,16 @@ __global__ __launch_bounds__(256) void energy_reduce_kernel(int nw, const zd* pe
   }
 }
 
-__global__ void site_reduce_kernel(EnergyArgs a) {
+// per-site densities weighted by den: walker chunks in parallel (blockIdx.y), then the chunk
+// partials summed in chunk order, so the result does not depend on scheduling
+__global__ void site_partial_kernel(EnergyArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.L) return;
   const int L = a.L;
+  const int per = (a.nw + SITE_CHUNKS - 1) / SITE_CHUNKS;
+  const int w0 = blockIdx.y * per, w1 = min(a.nw, w0 + per);
   zd su = cmk<double>(0, 0), sd = su, sp = su, sm = su;
-  for (int w = 0; w < a.nw; ++w) {
+  for (int w = w0; w < w1; ++w) {
     if (!a.alive[w]) continue;
     SiteG g = site_green(a.gd, a.go, a.ntile, a.Mb, a.dense, w, i, L);
     zd d = a.den[w];
@@ -422,10 +426,19 @@ __global__ void site_reduce_kernel(EnergyArgs a) {
     sp = cadd(sp, cmul(g.off_i, d));
     sm = cadd(sm, cmul(g.off_iL, d));
   }
-  a.site_acc[i] = cadd(a.site_acc[i], su);
-  a.site_acc[L + i] = cadd(a.site_acc[L + i], sd);
-  a.site_acc[2 * L + i] = cadd(a.site_acc[2 * L + i], sp);
-  a.site_acc[3 * L + i] = cadd(a.site_acc[3 * L + i], sm);
+  zd* o = a.site_part + (long long)blockIdx.y * 4 * L;
+  o[i] = su;
+  o[L + i] = sd;
+  o[2 * L + i] = sp;
+  o[3 * L + i] = sm;
+}
+
+__global__ void site_final_kernel(EnergyArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 4 * a.L) return;
+  zd s = cmk<double>(0, 0);
+  for (int c = 0; c < SITE_CHUNKS; ++c) s = cadd(s, a.site_part[(long long)c * 4 * a.L + e]);
+  a.site_acc[e] = cadd(a.site_acc[e], s);
 }
 
 __global__ void set_et_kernel(DevScalars* sc) {
@@ -508,7 +521,9 @@ int launch_energy(const EnergyArgs& a, hipStream_t s) {
   HIP_LAUNCH_CHECK();
   hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(256), 0, s, a.nw, a.per_walker, a.sc);
   HIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(site_reduce_kernel, dim3((a.L + 63) / 64), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(site_partial_kernel, dim3((a.L + 63) / 64, SITE_CHUNKS), dim3(64), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(site_final_kernel, dim3((4 * a.L + 255) / 256), dim3(256), 0, s, a);
   HIP_LAUNCH_CHECK();
   return 0;
 }

@@ -99,8 +99,10 @@ struct EnergyArgs {
   const zd* den;
   zd* per_walker;      // [nw][8]: den, H, K, V, R, Nup, Ndn, (unused)
   zd* site_acc;        // [4][L] += per site (Nup, Ndn, Splus, Sminus) * den
+  zd* site_part;       // [SITE_CHUNKS][4][L] scratch: per walker-chunk partial sums
   DevScalars* sc;
 };
+constexpr int SITE_CHUNKS = 64;
 int launch_energy(const EnergyArgs& a, hipStream_t s);
 int launch_set_et_from_energy(DevScalars* sc, hipStream_t s);
 int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int use_imag_sub,

@@ -25,13 +25,34 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
   __shared__ T As[2][BK][BM + PAD];
   __shared__ T Bs[2][BK][BN + PAD];
 
-  const int b = blockIdx.z;
+  // Workgroups are handed to the 8 XCDs in turn (workgroup id mod 8).  Renumber the tiles so
+  // that each XCD gets a contiguous range (M tile fastest, or N tile fastest when asked): the
+  // workgroups that share an operand panel then run on the same XCD and read it through one L2.
+  int bx, by, b;
+  {
+    const unsigned long long gx = gridDim.x, gy = gridDim.y;
+    const unsigned long long total = gx * gy * gridDim.z;
+    const unsigned long long L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned long long per = total / 8;
+    const unsigned long long lg = (L < per * 8) ? (L % 8) * per + L / 8 : L;
+    if (p.n_fastest) {
+      by = (int)(lg % gy);
+      const unsigned long long r = lg / gy;
+      bx = (int)(r % gx);
+      b = (int)(r / gx);
+    } else {
+      bx = (int)(lg % gx);
+      const unsigned long long r = lg / gx;
+      by = (int)(r % gy);
+      b = (int)(r / gy);
+    }
+  }
   if (p.alive && !p.alive[b]) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int m0 = bx * BM, n0 = by * BN;
 
   const ct* __restrict__ A = reinterpret_cast<const ct*>(p.A) + (long long)b * p.strideA;
   const T* __restrict__ Ar = reinterpret_cast<const T*>(p.A) + (long long)b * p.strideA;  // OP_NR
@@ -236,7 +257,7 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
         s1 = cadd(s1, red[1][w][m]);
         s2 = cadd(s2, red[2][w][m]);
       }
-      long long off = ((long long)b * ntn + blockIdx.y) * p.M + gm;
+      long long off = ((long long)b * ntn + by) * p.M + gm;
       p.out0[off] = s0;
       if (p.out1) p.out1[off] = s1;
       if (p.out2) p.out2[off] = s2;

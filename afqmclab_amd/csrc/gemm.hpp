@@ -30,6 +30,9 @@ struct GemmArgs {
   zd* out0; zd* out1; zd* out2;  // [batch][ntileN][M] partial sums
   // alive mask: batches with alive[b]==0 are skipped (batched mode only); nullptr = all
   const int* alive;
+  // tile order within an XCD: 0 = M tiles fastest (B panels shared), 1 = N tiles fastest (A panels
+  // shared; for a large shared A against many narrow B panels)
+  int n_fastest;
 };
 
 template <typename T>

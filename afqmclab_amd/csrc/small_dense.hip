@@ -2,10 +2,10 @@
 //
 // The N x N complex FP64 matrix (N <= 160; 112 x 112 is 196 KB, more than a
 // CU's LDS) is distributed 2D-cyclically over a TG x TG thread grid: thread
-// (tr, tc) owns rows tr + TG*a and columns tc + TG*b.  Only one column and one
-// row per elimination step travel through LDS (double buffered), pivots are
-// found with wavefront shuffles (every wave reduces redundantly, so no extra
-// barrier is needed for the pivot index).
+// (tr, tc) owns rows tr + TG*a and columns tc + TG*b.  Per elimination step one
+// column travels through LDS (double buffered) and the pivot row by in-wave
+// shuffle; the pivot of the next column is found one step ahead by the lanes
+// that own it, so each step has one barrier.
 //
 // gj_inverse_logdet: Gauss-Jordan exchange sweep with partial pivoting (the
 //   pivot of column k is the largest |re|+|im| among rows not yet used, first
@@ -17,188 +17,24 @@
 //   the walker restabilisation).
 #include "small_dense.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace {
 
 constexpr int NMAX = 160;
 
-__device__ __forceinline__ double abs1(zd a) { return fabs(a.re) + fabs(a.im); }
-
-template <typename T, int TG, int E>
-__global__ __launch_bounds__(TG * TG) void gj_kernel(SmallDenseArgs p) {
-  const int bidx = blockIdx.x;
-  if (p.alive && !p.alive[bidx / p.alive_div]) return;
-  const int N = p.N;
-  const int tid = threadIdx.x;
-  const int tr = tid % TG, tc = tid / TG;
-  const int lane = tid & 63;
-  typedef cplx<T> ct;
-  const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
-  ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
-
-  __shared__ zd colbuf[2][NMAX];
-  __shared__ zd pivs[NMAX];
-  __shared__ int perm[NMAX];
-  __shared__ int qinv[NMAX];
-  __shared__ unsigned char seen[NMAX];
-
-  zd x[E][E];
-#pragma unroll
-  for (int a = 0; a < E; ++a)
-#pragma unroll
-    for (int c = 0; c < E; ++c) {
-      int i = tr + TG * a, j = tc + TG * c;
-      x[a][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
-    }
-  // rows handled by this lane in the pivot search: lane, lane+64, lane+128
-  unsigned used = 0u;
-  int singular = 0;
-
-  for (int k = 0; k < N; ++k) {
-    const int buf = k & 1;
-    if (tc == k % TG) {
-      const int kc = k / TG;
-#pragma unroll
-      for (int a = 0; a < E; ++a) {
-        int i = tr + TG * a;
-#pragma unroll
-        for (int c = 0; c < E; ++c)
-          if (c == kc && i < N) colbuf[buf][i] = x[a][c];
-      }
-    }
-    __syncthreads();
-    // pivot search (redundant in every wave).  Key = bits of the non-negative score (monotone
-    // as an unsigned integer) with its low 8 bits replaced by 255 - row: the maximum key is the
-    // largest score, ties (to 2^-44 relative) going to the smallest row, as izamax does.
-    unsigned long long key = 0ull;
-#pragma unroll
-    for (int s = 0; s < (NMAX + 63) / 64; ++s) {
-      int i = lane + 64 * s;
-      if (i < N && !((used >> s) & 1u)) {
-        double v = abs1(colbuf[buf][i]);
-        if (!(v >= 0.0)) v = 0.0;  // NaN input: still pick a valid unused row (reported singular)
-        unsigned long long kk = ((unsigned long long)__double_as_longlong(v) & ~0xFFull) |
-                                (unsigned long long)(255 - i) | (1ull << 63);
-        key = kk > key ? kk : key;
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      unsigned long long ok = (unsigned long long)__shfl_xor((long long)key, o, 64);
-      key = ok > key ? ok : key;
-    }
-    // bit 63 marks a candidate; at step k there are N-k >= 1 unused rows, so the key is set
-    const int bi = 255 - (int)(key & 0xFFull);
-    const int pr = (bi >= 0 && bi < N) ? bi : 0;
-    const double bv = __longlong_as_double((long long)(key & ~(0xFFull | (1ull << 63))));
-    if ((pr & 63) == lane) used |= 1u << (pr >> 6);
-    const zd piv = colbuf[buf][pr];
-    zd inv = cmk<double>(0, 0);
-    if (bv > 0.0) {
-      inv = cinv(piv);
-    } else {
-      singular = 1;
-    }
-    if (tid == 0) {
-      perm[k] = pr;
-      pivs[k] = piv;
-    }
-    // Pivot row: thread (pr % TG, tc) owns row pr for this thread's columns, and it sits in
-    // the same wave (lane pr % TG + TG * (tc % (64 / TG))), so the row travels by shuffle.
-    zd rowv[E];
-    {
-      const int ar = pr / TG;
-      const int src = (pr % TG) + TG * (tc % (64 / TG));
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        zd v = cmk<double>(0, 0);
-#pragma unroll
-        for (int a = 0; a < E; ++a)
-          if (a == ar) v = x[a][c];
-        v.re = __shfl(v.re, src, 64);
-        v.im = __shfl(v.im, src, 64);
-        const int j = tc + TG * c;
-        rowv[c] = (j == k) ? cmk<double>(0, 0) : cmul(v, inv);
-      }
-    }
-    // rank-1 update of every element (rowv[k] = 0 leaves column k alone; padded rows and
-    // columns beyond N compute on values that are never stored), then the pivot row and the
-    // pivot column are overwritten by the threads that own them
-    zd cv[E];
-#pragma unroll
-    for (int a = 0; a < E; ++a) cv[a] = colbuf[buf][tr + TG * a];
-#pragma unroll
-    for (int a = 0; a < E; ++a)
-#pragma unroll
-      for (int c = 0; c < E; ++c) x[a][c] = csub(x[a][c], cmul(cv[a], rowv[c]));
-    if (tr == pr % TG) {
-      const int ar = pr / TG;
-#pragma unroll
-      for (int a = 0; a < E; ++a)
-        if (a == ar)
-#pragma unroll
-          for (int c = 0; c < E; ++c) x[a][c] = cmk<double>(-rowv[c].re, -rowv[c].im);
-    }
-    if (tc == k % TG) {
-      const int kc = k / TG;
-#pragma unroll
-      for (int c = 0; c < E; ++c)
-        if (c == kc)
-#pragma unroll
-          for (int a = 0; a < E; ++a) x[a][c] = (tr + TG * a == pr) ? inv : cmul(cv[a], inv);
-    }
-  }
-  __syncthreads();
-  if (tid < N) qinv[perm[tid]] = tid;
-  if (tid == 0) {
-    // log|det| and the phase product in pivot order, off the elimination's critical path
-    double lognorm = 0.0;
-    zd phase = cmk<double>(1.0, 0.0);
-    if (!singular) {
-      for (int k = 0; k < N; ++k) {
-        const zd pv = pivs[k];
-        const double ap = cabs(pv);
-        lognorm += log(ap);
-        phase = cmul(phase, cmk<double>(pv.re / ap, pv.im / ap));
-      }
-    }
-    // sign of the permutation k -> perm[k]
-    int visited_parity = 0;
-    for (int i = 0; i < N; ++i) seen[i] = 0;
-    for (int i = 0; i < N; ++i) {
-      if (seen[i]) continue;
-      int len = 0, j = i;
-      // perm is a permutation of 0..N-1 (each row is used once), so a cycle has at most N
-      // elements; the length bound keeps the walk finite even if that were violated
-      while (len < N && !seen[j]) { seen[j] = 1; j = perm[j]; ++len; }
-      visited_parity ^= (len - 1) & 1;
-    }
-    zd ph = visited_parity ? cmk<double>(-phase.re, -phase.im) : phase;
-    zd ld;
-    if (singular) {
-      ld = cmk<double>(-1e300, 0.0);
-    } else {
-      ld = cmk<double>(lognorm + 0.0, atan2(ph.im, ph.re));
-      // log(phase) of a unit complex: log|ph| is rounding-level; keep it as the reference does
-      ld.re += log(hypot(ph.re, ph.im));
-    }
-    p.logdet[bidx] = ld;
-    if (p.info) p.info[bidx] = singular;
-  }
-  if (!p.write_inverse) return;
-  __syncthreads();
-#pragma unroll
-  for (int a = 0; a < E; ++a) {
-    int i = tr + TG * a;
-    if (i >= N) continue;
-    int row = qinv[i];
-#pragma unroll
-    for (int c = 0; c < E; ++c) {
-      int j = tc + TG * c;
-      if (j >= N) continue;
-      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(x[a][c]);
-    }
-  }
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, E-1>)
+template <int E, typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
 }
+template <int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl<E>(f, std::make_integer_sequence<int, E>{});
+}
+
+__device__ __forceinline__ double abs1(zd a) { return fabs(a.re) + fabs(a.im); }
 
 // Packed pivot key of a candidate row: the bits of the non-negative score |re|+|im| (monotone as
 // an unsigned integer) with the low 8 bits replaced by 255 - row, and bit 63 set.  The maximum
@@ -209,11 +45,17 @@ __device__ __forceinline__ unsigned long long pivot_key(zd v, int row) {
   return ((unsigned long long)__double_as_longlong(s) & ~0xFFull) | (unsigned long long)(255 - row) | (1ull << 63);
 }
 
+// x - a*b with two fused multiply-adds per component
+__device__ __forceinline__ zd cfms(zd x, zd a, zd b) {
+  return cmk<double>(fma(-a.re, b.re, fma(a.im, b.im, x.re)), fma(-a.re, b.im, fma(-a.im, b.re, x.im)));
+}
+
 // Gauss-Jordan with look-ahead: the threads that own column k+1 (TG consecutive lanes of one
 // wave) update that column first during step k, find its pivot with a reduction inside their
 // lane group and publish the column and the pivot row through LDS, then do the rest of their
 // update.  The pivot search is thereby off the critical path of the other waves, and each
-// step has one barrier.  Same pivots, logdet and inverse as the sweep above.
+// step has one barrier.  The pivot row index is uniform over the workgroup (read as a scalar),
+// so the pivot row is taken from registers by a scalar branch and an in-wave shuffle.
 template <typename T, int TG, int E>
 __global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
   const int bidx = blockIdx.x;
@@ -227,6 +69,7 @@ __global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
 
   __shared__ zd colbuf[2][NMAX];
   __shared__ int pivrow[2];
+  __shared__ zd pivinv[2];
   __shared__ zd pivs[NMAX];
   __shared__ int perm[NMAX];
   __shared__ int qinv[NMAX];
@@ -243,18 +86,16 @@ __global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
   unsigned used = 0u;  // bit a: row tr + TG*a has been a pivot row
   int singular = 0;
 
-  // the owners of column `col` publish it and its pivot row (reduction over their TG lanes)
-  auto publish = [&](int col, int buf) {
-    const int cc = col / TG;
+  // The owners of column `col` (TG consecutive lanes) publish it: value(a) gives their entry in
+  // register row a; the pivot row is found by a reduction inside the lane group, and the lane
+  // holding the pivot publishes its index and inverse (0 for a zero pivot).
+  auto publish_col = [&](auto value, int col, int buf) {
     unsigned long long key = 0ull;
 #pragma unroll
     for (int a = 0; a < E; ++a) {
       const int i = tr + TG * a;
-      zd v = cmk<double>(0, 0);
-#pragma unroll
-      for (int c = 0; c < E; ++c)
-        if (c == cc) v = x[a][c];
       if (i < N) {
+        const zd v = value(a);
         colbuf[buf][i] = v;
         if (!((used >> a) & 1u)) {
           const unsigned long long kk = pivot_key(v, i);
@@ -267,46 +108,52 @@ __global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
       const unsigned long long ok = (unsigned long long)__shfl_xor((long long)key, o, 64);
       key = ok > key ? ok : key;
     }
-    if (tr == 0) {
-      // bit 63 marks a candidate: at step col there are N - col >= 1 unused rows
-      const int bi = 255 - (int)(key & 0xFFull);
-      pivrow[buf] = (bi >= 0 && bi < N) ? bi : 0;
+    // bit 63 marks a candidate: at step col there are N - col >= 1 unused rows
+    const int bi = 255 - (int)(key & 0xFFull);
+    const int prn = (bi >= 0 && bi < N) ? bi : 0;
+    if (tr == prn % TG) {
+      const zd pv = colbuf[buf][prn];  // written by this lane above
+      pivrow[buf] = prn;
+      pivinv[buf] = abs1(pv) > 0.0 ? cinv(pv) : cmk<double>(0, 0);
     }
   };
 
-  if (tc == 0) publish(0, 0);
+  if (tc == 0) publish_col([&](int a) { return x[a][0]; }, 0, 0);
   __syncthreads();
 
-  for (int k = 0; k < N; ++k) {
+  // Step k = CB*TG + t: column k sits in register column CB (a compile-time constant), column
+  // k+1 in CB or CB+1, so no register array is ever indexed by a run-time value.
+  auto step = [&](auto cbc, int t) {
+    constexpr int CB = decltype(cbc)::value;
+    const int k = CB * TG + t;
     const int buf = k & 1;
-    const int pr = pivrow[buf];
+    const int pr = __builtin_amdgcn_readfirstlane(pivrow[buf]);
     const zd piv = colbuf[buf][pr];
-    zd inv = cmk<double>(0, 0);
-    if (abs1(piv) > 0.0) {
-      inv = cinv(piv);
-    } else {
-      singular = 1;
-    }
+    const zd inv = pivinv[buf];
+    if (!(abs1(piv) > 0.0)) singular = 1;
     if (tid == 0) {
       perm[k] = pr;
       pivs[k] = piv;
     }
-    const bool prow = (tr == pr % TG);
-    const int ar = pr / TG;
+    const int ar = pr / TG, prm = pr % TG;
+    const bool prow = (tr == prm);
     if (prow) used |= 1u << ar;
+    // rowv[c] = x[pr][j] / piv (ar is uniform, so these are scalar branches around the shuffles);
+    // the update below leaves column k to its own fix-up
     zd rowv[E];
     {
-      const int src = (pr % TG) + TG * (tc % (64 / TG));
+      const int src = prm + TG * (tc % (64 / TG));
 #pragma unroll
-      for (int c = 0; c < E; ++c) {
-        zd v = cmk<double>(0, 0);
+      for (int a = 0; a < E; ++a) {
+        if (a == ar) {
 #pragma unroll
-        for (int a = 0; a < E; ++a)
-          if (a == ar) v = x[a][c];
-        v.re = __shfl(v.re, src, 64);
-        v.im = __shfl(v.im, src, 64);
-        const int j = tc + TG * c;
-        rowv[c] = (j == k) ? cmk<double>(0, 0) : cmul(v, inv);
+          for (int c = 0; c < E; ++c) {
+            zd v;
+            v.re = __shfl(x[a][c].re, src, 64);
+            v.im = __shfl(x[a][c].im, src, 64);
+            rowv[c] = cmul(v, inv);
+          }
+        }
       }
     }
     zd cv[E];
@@ -315,30 +162,26 @@ __global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
       const int i = tr + TG * a;
       cv[a] = i < N ? colbuf[buf][i] : cmk<double>(0, 0);
     }
-    // look-ahead: column k+1 first (its owners), then publish it
-    const bool ahead = (k + 1 < N) && (tc == (k + 1) % TG);
-    const int c1 = (k + 1) / TG;
-    if (ahead) {
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        if (c != c1) continue;
-#pragma unroll
-        for (int a = 0; a < E; ++a) x[a][c] = csub(x[a][c], cmul(cv[a], rowv[c]));
-        if (prow) {
-#pragma unroll
-          for (int a = 0; a < E; ++a)
-            if (a == ar) x[a][c] = cmk<double>(-rowv[c].re, -rowv[c].im);
-        }
+    // look-ahead: the owners of column k+1 compute its updated values first and publish them
+    // (the update below recomputes the same values in place)
+    auto lookahead = [&](auto c1c) {
+      constexpr int C1 = decltype(c1c)::value;
+      publish_col([&](int a) {
+        return (prow && a == ar) ? cmk<double>(-rowv[C1].re, -rowv[C1].im) : cfms(x[a][C1], cv[a], rowv[C1]);
+      }, k + 1, buf ^ 1);
+    };
+    if (k + 1 < N && tc == (t + 1) % TG) {
+      if (t + 1 < TG) {
+        lookahead(std::integral_constant<int, CB>{});
+      } else {
+        if constexpr (CB + 1 < E) lookahead(std::integral_constant<int, CB + 1>{});
       }
-      publish(k + 1, buf ^ 1);
     }
-    // the rest of the rank-1 update, then the pivot row and column k
+    // rank-1 update, then the pivot row and column k
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      if (ahead && c == c1) continue;
+    for (int c = 0; c < E; ++c)
 #pragma unroll
-      for (int a = 0; a < E; ++a) x[a][c] = csub(x[a][c], cmul(cv[a], rowv[c]));
-    }
+      for (int a = 0; a < E; ++a) x[a][c] = cfms(x[a][c], cv[a], rowv[c]);
     if (prow) {
 #pragma unroll
       for (int a = 0; a < E; ++a)
@@ -346,16 +189,17 @@ __global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
 #pragma unroll
           for (int c = 0; c < E; ++c) x[a][c] = cmk<double>(-rowv[c].re, -rowv[c].im);
     }
-    if (tc == k % TG) {
-      const int kc = k / TG;
+    if (tc == t) {
 #pragma unroll
-      for (int c = 0; c < E; ++c)
-        if (c == kc)
-#pragma unroll
-          for (int a = 0; a < E; ++a) x[a][c] = (tr + TG * a == pr) ? inv : cmul(cv[a], inv);
+      for (int a = 0; a < E; ++a) x[a][CB] = (tr + TG * a == pr) ? inv : cmul(cv[a], inv);
     }
     __syncthreads();
-  }
+  };
+  auto block = [&](auto cbc) {
+    constexpr int CB = decltype(cbc)::value;
+    for (int t = 0; t < TG && CB * TG + t < N; ++t) step(cbc, t);
+  };
+  static_for<E>(block);
   if (tid < N) qinv[perm[tid]] = tid;
   if (tid == 0) {
     double lognorm = 0.0;
@@ -475,7 +319,7 @@ __global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
       if (j < i) x[a][c] = cmk<double>(0, 0);
     }
   // R^{-1} by the exchange sweep with pivot (k, k); the pivot row reaches each thread by an
-  // in-wave shuffle (see gj_kernel), so one barrier per step
+  // in-wave shuffle (as in gj_la_kernel), so one barrier per step
   __syncthreads();
   for (int k = 0; k < N; ++k) {
     const int buf = k & 1;
@@ -546,7 +390,7 @@ __global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
 template <typename T, int TG, int E>
 int run(int kind, const SmallDenseArgs& a, hipStream_t s) {
   dim3 grid(a.batch), block(TG * TG);
-  if (kind == 0) hipLaunchKernelGGL((gj_kernel<T, TG, E>), grid, block, 0, s, a);
+  if (kind == 0) hipLaunchKernelGGL((gj_la_kernel<T, TG, E>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((chol_kernel<T, TG, E>), grid, block, 0, s, a);
   HIP_LAUNCH_CHECK();
   return 0;

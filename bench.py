@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ph-steps", type=int, default=60, help="phaseless leg steps (0 = skip)")
     ap.add_argument("--ph-walkers", type=int, default=1024)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="first run a plain device copy of BYTES (reference kernel for HBM traffic counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -213,6 +215,16 @@ def main():
     sched = dict(dt=0.005, mgs=10, pop=40, thermal=100, et_step=10, et_max=100, meas=5)
     model, phi = build_workload(nx, ny, nup, ndn, args.U)
     ctx = Context(0)
+    copy_cal = None
+    if args.calibrate_copy > 0:
+        import ctypes as C
+        from afqmclab_amd._lib import check, lib
+        ms = C.c_double()
+        check(lib().afqmc_test_copy(ctx.h, args.calibrate_copy, 3, C.byref(ms)))
+        copy_cal = {"bytes_read": args.calibrate_copy, "bytes_written": args.calibrate_copy, "ms": ms.value,
+                    "GB_per_s": 2 * args.calibrate_copy / (ms.value * 1e-3) / 1e9}
+        log(f"[bench] copy calibration: {args.calibrate_copy} B in {ms.value:.3f} ms "
+            f"({copy_cal['GB_per_s']:.0f} GB/s read+write)")
     results = {}
     for path in args.paths.split(","):
         precision = 1 if path == "mixed" else 0
@@ -256,6 +268,17 @@ def main():
                      "avg_launch_ms": avg_launch_s * 1e3, "flops_per_launch": per_launch_flops},
         "paths": {},
     }
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes scaled by the same counters on a plain copy (profiles/)
+    tpath = os.path.join(ROOT, "profiles", "cp_propagate_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("workload_walkers") == args.walkers and tr.get("path") == main_path:
+            out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+            out["roofline"]["algorithmic_bytes"] = tr.get("algorithmic_bytes_per_launch")
+    if copy_cal:
+        out["copy_calibration"] = copy_cal
     for p, rr in results.items():
         out["paths"][p] = {
             "walker_steps_per_s": rr["walker_steps"] / rr["elapsed"],

@@ -238,6 +238,9 @@ int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K,
 /* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting) */
 int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,
                           double* logdet, int* info);
+/* device-to-device streaming copy of `bytes` (a multiple of 16), reps times: the reference kernel
+ * for scaling HBM traffic counters; ms = average kernel time */
+int afqmc_test_copy(afqmc_ctx ctx, long long bytes, int reps, double* ms);
 /* batched Cholesky S = R^H R, returns R^{-1} and (sum log R_kk, same) */
 int afqmc_test_chol_inverse(afqmc_ctx ctx, int N, int batch, const double* S, double* Rinv, double* logsum,
                             int* info);
