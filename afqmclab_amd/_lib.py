@@ -50,6 +50,14 @@ SIGNATURES = {
     "afqmc_ctx_synchronize": (C.c_int, [_vp]),
     "afqmc_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
     "afqmc_ctx_get_timing": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int]),
+    "afqmc_capture_begin": (C.c_int, [_vp]),
+    "afqmc_capture_begin_group": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "afqmc_capture_end": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "afqmc_graph_launch": (C.c_int, [_vp]),
+    "afqmc_graph_node_count": (C.c_int, [_vp]),
+    "afqmc_graph_destroy": (C.c_int, [_vp]),
+    "afqmc_cp_set_tiles": (C.c_int, [_vp, _ip]),
+    "afqmc_cp_autotune": (C.c_int, [_vp, _vp, C.c_int, _ip, _dp]),
     "afqmc_cp_create": (C.c_int, [_vp, C.POINTER(CPDesc), C.POINTER(_vp)]),
     "afqmc_cp_destroy": (C.c_int, [_vp]),
     "afqmc_cp_get_hs_constants": (C.c_int, [_vp, _dp]),

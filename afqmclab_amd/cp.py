@@ -44,9 +44,49 @@ class Context:
         check(lib().afqmc_ctx_get_timing(self.h, dptr(ms), cnt, n))
         return {r: (float(ms[i]), int(cnt[i])) for i, r in enumerate(self.REGIONS)}
 
+    # ---- HIP-graph capture of enqueue-only calls --------------------------
+    def capture_begin(self, members=()):
+        """Record (instead of run) the enqueue-only calls made until capture_end(); calls on the
+        member contexts become parallel branches of the same graph."""
+        members = list(members)
+        if members:
+            arr = (C.c_void_p * len(members))(*[m.h for m in members])
+            check(lib().afqmc_capture_begin_group(self.h, len(members), arr))
+        else:
+            check(lib().afqmc_capture_begin(self.h))
+
+    def capture_end(self):
+        g = C.c_void_p()
+        check(lib().afqmc_capture_end(self.h, C.byref(g)))
+        return Graph(g)
+
     def close(self):
         if self.h:
             lib().afqmc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """An instantiated HIP graph of captured calls; launch() enqueues it (asynchronous)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self):
+        check(lib().afqmc_graph_launch(self.h))
+
+    def nodes(self):
+        return lib().afqmc_graph_node_count(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().afqmc_graph_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -203,6 +243,21 @@ class CPPropagator:
 
     def walkers(self, nw):
         return WalkerSet(self, nw)
+
+    # ---- GEMM tile shapes -------------------------------------------------
+    TILES = ["64x64", "32x64", "64x32", "32x32"]
+
+    def set_tiles(self, propagate=0, overlap=0, green=0):
+        t = np.array([propagate, overlap, green], dtype=np.int32)
+        check(lib().afqmc_cp_set_tiles(self.h, iptr(t)))
+
+    def autotune(self, w, reps=5):
+        """Time every tile shape on this walker set's propagate / overlap / Green GEMMs, keep the
+        fastest; returns (tiles, ms[3][4])."""
+        t = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(3 * 4)
+        check(lib().afqmc_cp_autotune(self.h, w.h, int(reps), iptr(t), dptr(ms)))
+        return t, ms.reshape(3, 4)
 
     # ---- batched drop-ins -------------------------------------------------
     def log_overlap(self, w):

@@ -85,6 +85,7 @@ struct afqmc_cp_s {
   DevBuf<double> weight, comb_prob, force_rec;
   DevBuf<int> aux_rec;
   int ntile[2] = {1, 1};
+  int tile[3] = {GEMM_TILE_64x64, GEMM_TILE_64x64, GEMM_TILE_64x64};  // propagate, overlap, green
   std::vector<cd> h_gamma, h_cdiag;
   size_t esz() const { return fp32 ? sizeof(zf) : sizeof(zd); }
 };
@@ -150,6 +151,7 @@ int overlap_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf) {
     a.A = cp->phiT[b].p; a.lda = cp->Mb[b]; a.strideA = 0;
     a.B = wf[b].p; a.ldb = cp->Mb[b]; a.strideB = 0;
     a.C = cp->ovl[b].p; a.ldc = cp->Nb[b]; a.strideC = 0;
+    a.tile = cp->tile[1];
     int rc = cp_gemm<T>(cp, OP_C, EPI_STORE, a);
     if (rc) return rc;
   }
@@ -185,6 +187,7 @@ int green_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, const int* alive,
     a.out0 = cp->gpart[0][b].p; a.out1 = cp->dense ? cp->gpart[1][b].p : nullptr;
     a.out2 = kinetic ? cp->gpart[2][b].p : nullptr;
     a.alive = alive;
+    a.tile = cp->tile[2];
     int rc = cp_gemm<T>(cp, OP_N, EPI_GREEN, a);
     if (rc) return rc;
   }
@@ -203,6 +206,7 @@ int propagate_gemm(afqmc_cp cp, afqmc_walkers w, int which, DevBuf<char>* in, De
     if (scaled) {
       a.scale = cp->scale[b].p; a.scale_group = cp->Nb[b]; a.scale_strideG = cp->Mb[b]; a.scale_strideB = 0;
     }
+    a.tile = cp->tile[0];
     int rc = cp_gemm<T>(cp, OP_N, EPI_STORE, a);
     if (rc) return rc;
   }
@@ -251,6 +255,8 @@ template <typename T>
 int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out) {
   hipStream_t s = cp->ctx->stream;
   int rc;
+  REQUIRE(!capturing() || (!force_out && !aux_out), "a captured step cannot return records");
+  for (int b = 0; b < cp->nblk; ++b) capture_note_pair(&w->wf[b], &w->tmp[b]);
   {
     RegionTimer t(cp->ctx, REG_OVERLAP);
     if ((rc = overlap_gemm<T>(cp, w, w->wf))) return rc;
@@ -332,6 +338,10 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
 
 int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
   RegionTimer t(cp->ctx, REG_QR);
+  for (int b = 0; b < cp->nblk; ++b) {
+    capture_note_pair(&w->wf[b], &w->tmp[b]);
+    capture_note_pair(&cp->qr64[b], &cp->qr64b[b]);
+  }
   if (!cp->fp32) return qr_impl_f64(cp, w, w->wf, w->tmp);
   // mixed path: QR in FP64 on a widened copy of the FP32 walkers
   hipStream_t s = cp->ctx->stream;
@@ -401,9 +411,48 @@ int measure_impl(afqmc_cp cp, afqmc_walkers w) {
 #define DISPATCH(cp, fn, ...) ((cp)->fp32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
 
 // ------------------------------------------------------------------ C entry points
+namespace {
+template <typename T>
+int tune_role(afqmc_cp cp, afqmc_walkers w, int role, int reps, double* ms_row) {
+  hipStream_t s = cp->ctx->stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double best = 1e300;
+  int best_tile = cp->tile[role];
+  int rc = 0;
+  for (int t = 0; t < GEMM_TILE_COUNT && !rc; ++t) {
+    ms_row[t] = -1.0;
+    if (role == 2 && gemm_tile_bn(t) != 64) continue;
+    cp->tile[role] = t;
+    auto run = [&]() {
+      // each role reads the walkers and writes only workspace (tmp, overlaps, Green partials)
+      if (role == 0) return propagate_gemm<T>(cp, w, 0, w->wf, w->tmp, cp->diag_only != 0);
+      if (role == 1) return overlap_gemm<T>(cp, w, w->wf);
+      return green_gemm<T>(cp, w, w->wf, w->alive.p, false);
+    };
+    if ((rc = run())) break;
+    if (hipEventRecord(e0, s) != hipSuccess) { rc = -2; break; }
+    for (int r = 0; r < reps && !rc; ++r) rc = run();
+    if (rc) break;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -2; break; }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    ms_row[t] = ms / reps;
+    if (ms_row[t] < best) { best = ms_row[t]; best_tile = t; }
+  }
+  cp->tile[role] = best_tile;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc == -2) afqmc_set_error("afqmc_cp_autotune: event failure");
+  return rc;
+}
+}  // namespace
+
 extern "C" {
 
 int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
+  NOT_CAPTURING();
   REQUIRE(ctx && d && out, "afqmc_cp_create: null argument");
   REQUIRE(d->L > 0 && d->N > 0 && d->max_walkers > 0, "afqmc_cp_create: L, N, max_walkers must be positive");
   REQUIRE(d->K && d->exp_minus_dt_k && d->exp_minus_half_dt_k && d->exp_half_dt_k && d->phiT && d->U,
@@ -565,6 +614,7 @@ int afqmc_cp_destroy(afqmc_cp cp) {
 }
 
 int afqmc_cp_get_hs_constants(afqmc_cp cp, double* out) {
+  NOT_CAPTURING();
   REQUIRE(cp && out, "null argument");
   const int L = cp->L;
   for (int i = 0; i < L; ++i) { out[2 * i] = cp->h_gamma[i].real(); out[2 * i + 1] = cp->h_gamma[i].imag(); }
@@ -576,6 +626,7 @@ int afqmc_cp_get_hs_constants(afqmc_cp cp, double* out) {
 }
 
 int afqmc_cp_walkers_create(afqmc_cp cp, int nw, afqmc_walkers* out) {
+  NOT_CAPTURING();
   REQUIRE(cp && out, "null argument");
   REQUIRE(nw > 0 && nw <= cp->maxw, "walker count %d outside 1..max_walkers=%d", nw, cp->maxw);
   ScopedDevice sd(cp->ctx);
@@ -618,6 +669,7 @@ int afqmc_walkers_destroy(afqmc_walkers w) {
 int afqmc_walkers_count(afqmc_walkers w) { return w ? w->nw : 0; }
 
 int afqmc_walkers_upload(afqmc_walkers w, const double* wf, const double* logw, const int* alive) {
+  NOT_CAPTURING();
   REQUIRE(w && wf && logw, "null argument");
   afqmc_cp cp = w->cp;
   ScopedDevice sd(cp->ctx);
@@ -659,6 +711,7 @@ int afqmc_walkers_upload(afqmc_walkers w, const double* wf, const double* logw, 
 }
 
 int afqmc_walkers_download(afqmc_walkers w, double* wf, double* logw, int* alive) {
+  NOT_CAPTURING();
   REQUIRE(w, "null argument");
   afqmc_cp cp = w->cp;
   ScopedDevice sd(cp->ctx);
@@ -693,6 +746,7 @@ int afqmc_walkers_download(afqmc_walkers w, double* wf, double* logw, int* alive
 }
 
 int afqmc_cp_set_seed(afqmc_cp cp, uint64_t seed, uint64_t counter) {
+  NOT_CAPTURING();
   REQUIRE(cp, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -705,6 +759,7 @@ int afqmc_cp_set_seed(afqmc_cp cp, uint64_t seed, uint64_t counter) {
 }
 
 int afqmc_cp_get_counter(afqmc_cp cp, uint64_t* counter) {
+  NOT_CAPTURING();
   REQUIRE(cp && counter, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -716,6 +771,7 @@ int afqmc_cp_get_counter(afqmc_cp cp, uint64_t* counter) {
 }
 
 int afqmc_cp_get_walker_steps(afqmc_cp cp, uint64_t* walker_steps) {
+  NOT_CAPTURING();
   REQUIRE(cp && walker_steps, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -726,6 +782,7 @@ int afqmc_cp_get_walker_steps(afqmc_cp cp, uint64_t* walker_steps) {
 }
 
 int afqmc_cp_set_uniforms(afqmc_cp cp, const double* u, int64_t n) {
+  NOT_CAPTURING();
   REQUIRE(cp && n >= 0, "invalid argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -747,6 +804,7 @@ int afqmc_cp_set_uniforms(afqmc_cp cp, const double* u, int64_t n) {
 }
 
 int afqmc_cp_skip_uniforms(afqmc_cp cp, uint64_t n) {
+  NOT_CAPTURING();
   REQUIRE(cp, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -758,6 +816,7 @@ int afqmc_cp_skip_uniforms(afqmc_cp cp, uint64_t n) {
 }
 
 int afqmc_cp_set_ET(afqmc_cp cp, double ET) {
+  NOT_CAPTURING();
   REQUIRE(cp, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -766,6 +825,7 @@ int afqmc_cp_set_ET(afqmc_cp cp, double ET) {
 }
 
 int afqmc_cp_get_ET(afqmc_cp cp, double* ET) {
+  NOT_CAPTURING();
   REQUIRE(cp && ET, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -774,6 +834,7 @@ int afqmc_cp_get_ET(afqmc_cp cp, double* ET) {
 }
 
 int afqmc_cp_log_overlap(afqmc_cp cp, afqmc_walkers w, double* logO) {
+  NOT_CAPTURING();
   REQUIRE(cp && w && logO && w->cp == cp, "invalid argument");
   ScopedDevice sd(cp->ctx);
   int rc = cp->fp32 ? do_overlap<float>(cp, w, false) : do_overlap<double>(cp, w, false);
@@ -786,6 +847,7 @@ int afqmc_cp_log_overlap(afqmc_cp cp, afqmc_walkers w, double* logO) {
 }
 
 int afqmc_cp_green_diagonal(afqmc_cp cp, afqmc_walkers w, double* gdiag, double* goff) {
+  NOT_CAPTURING();
   REQUIRE(cp && w && gdiag && w->cp == cp, "invalid argument");
   ScopedDevice sd(cp->ctx);
   int rc = cp->fp32 ? do_overlap<float>(cp, w, true) : do_overlap<double>(cp, w, true);
@@ -825,6 +887,7 @@ int afqmc_cp_green_diagonal(afqmc_cp cp, afqmc_walkers w, double* gdiag, double*
 }
 
 int afqmc_cp_force(afqmc_cp cp, afqmc_walkers w, double* force) {
+  NOT_CAPTURING();
   REQUIRE(cp && w && force && w->cp == cp, "invalid argument");
   const int L = cp->L, L2 = 2 * L;
   std::vector<double> gd((size_t)w->nw * L2 * 2), go((size_t)w->nw * L2 * 2);
@@ -860,11 +923,12 @@ int afqmc_cp_onebody(afqmc_cp cp, afqmc_walkers w, int which) {
       return rc;
   }
   if ((rc = launch_logw_const_add(w->nw, w->logw.p, w->alive.p, cp->hop_logw[which], cp->ctx->stream))) return rc;
-  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  HIP_TRY(sync_unless_capturing(cp->ctx->stream));
   return 0;
 }
 
 int afqmc_cp_hs_apply(afqmc_cp cp, afqmc_walkers w, const double* diag4, const double* sample_logw) {
+  NOT_CAPTURING();
   REQUIRE(cp && w && diag4 && sample_logw && w->cp == cp, "invalid argument");
   REQUIRE(cp->dense, "afqmc_cp_hs_apply needs a dense (spin_blocks = 0) set");
   ScopedDevice sd(cp->ctx);
@@ -898,11 +962,12 @@ int afqmc_cp_stabilize(afqmc_cp cp, afqmc_walkers w) {
   ScopedDevice sd(cp->ctx);
   int rc = stabilize_impl(cp, w);
   if (rc) return rc;
-  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  HIP_TRY(sync_unless_capturing(cp->ctx->stream));
   return 0;
 }
 
 int afqmc_comb(afqmc_ctx ctx, int nw, const double* weight, const double* uniforms, int* table, double* stats) {
+  NOT_CAPTURING();
   REQUIRE(ctx && weight && uniforms && table && nw > 0, "invalid argument");
   ScopedDevice sd(ctx);
   hipStream_t s = ctx->stream;
@@ -931,6 +996,7 @@ int afqmc_comb(afqmc_ctx ctx, int nw, const double* weight, const double* unifor
 }
 
 int afqmc_cp_apply_table(afqmc_cp cp, afqmc_walkers w, const int* table) {
+  NOT_CAPTURING();
   REQUIRE(cp && w && table && w->cp == cp, "invalid argument");
   for (int i = 0; i < w->nw; ++i) REQUIRE(table[i] >= 0 && table[i] < w->nw, "table entry %d out of range", i);
   ScopedDevice sd(cp->ctx);
@@ -953,6 +1019,7 @@ int afqmc_cp_step(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out)
 
 int afqmc_cp_pop_control(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats) {
   REQUIRE(cp && w && w->cp == cp, "invalid argument");
+  REQUIRE(!capturing() || (!weights_out && !table_out && !stats), "a captured comb cannot return outputs");
   ScopedDevice sd(cp->ctx);
   return DISPATCH(cp, pop_control_impl, cp, w, weights_out, table_out, stats);
 }
@@ -964,6 +1031,7 @@ int afqmc_cp_measure(afqmc_cp cp, afqmc_walkers w) {
 }
 
 int afqmc_cp_energy(afqmc_cp cp, double* out) {
+  NOT_CAPTURING();
   REQUIRE(cp && out, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -975,6 +1043,7 @@ int afqmc_cp_energy(afqmc_cp cp, double* out) {
 }
 
 int afqmc_cp_site_measure(afqmc_cp cp, double* out) {
+  NOT_CAPTURING();
   REQUIRE(cp && out, "null argument");
   ScopedDevice sd(cp->ctx);
   HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
@@ -996,6 +1065,33 @@ int afqmc_cp_adjust_ET(afqmc_cp cp) {
   int rc = launch_set_et_from_energy(cp->sc.p, cp->ctx->stream);
   if (rc) return rc;
   return launch_reset_measure(cp->sc.p, cp->site_acc.p, cp->L, cp->ctx->stream);
+}
+
+int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles) {
+  REQUIRE(cp && tiles, "null argument");
+  for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= 0 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
+  REQUIRE(gemm_tile_bn(tiles[2]) == 64, "the Green GEMM takes tile 0 or 1");
+  for (int r = 0; r < 3; ++r) cp->tile[r] = tiles[r];
+  return 0;
+}
+
+
+int afqmc_cp_autotune(afqmc_cp cp, afqmc_walkers w, int reps, int* tiles, double* ms) {
+  REQUIRE(cp && w && w->cp == cp && reps > 0, "invalid argument");
+  NOT_CAPTURING();
+  ScopedDevice sd(cp->ctx);
+  double t[3][GEMM_TILE_COUNT];
+  int rc = 0;
+  for (int role = 0; role < 3 && !rc; ++role)
+    rc = cp->fp32 ? tune_role<float>(cp, w, role, reps, t[role]) : tune_role<double>(cp, w, role, reps, t[role]);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  if (tiles)
+    for (int r = 0; r < 3; ++r) tiles[r] = cp->tile[r];
+  if (ms)
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < GEMM_TILE_COUNT; ++k) ms[r * GEMM_TILE_COUNT + k] = t[r][k];
+  return 0;
 }
 
 }  // extern "C"

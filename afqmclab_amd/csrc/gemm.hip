@@ -279,8 +279,25 @@ int run(const GemmArgs& a, hipStream_t s) {
 
 template <typename T, int OPA, int EPI, bool SCALE>
 int dispatch_tile(const GemmArgs& a, hipStream_t s) {
-  // 64x64 output tiles, 4 waves (2x2), K tile 16: 32 accumulator values per lane
+  // 4 waves (2 x 2), K tile 16; 64 x 64 holds 32 accumulator values per lane
+  switch (a.tile) {
+    case GEMM_TILE_32x64: return run<T, OPA, 32, 64, 16, 2, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_64x32: return run<T, OPA, 64, 32, 16, 2, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_32x32: return run<T, OPA, 32, 32, 16, 2, 2, EPI, SCALE>(a, s);
+    default: return run<T, OPA, 64, 64, 16, 2, 2, EPI, SCALE>(a, s);
+  }
+}
+
+// call sites without tuned shapes use the default tile only (fewer instantiations)
+template <typename T, int OPA, int EPI, bool SCALE>
+int dispatch_default(const GemmArgs& a, hipStream_t s) {
   return run<T, OPA, 64, 64, 16, 2, 2, EPI, SCALE>(a, s);
+}
+
+template <typename T>
+int dispatch_green(const GemmArgs& a, hipStream_t s) {
+  if (a.tile == GEMM_TILE_32x64) return run<T, OP_N, 32, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
+  return run<T, OP_N, 64, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
 }
 
 }  // namespace
@@ -292,13 +309,15 @@ int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s) {
   const bool sc = a.scale != nullptr;
   REQUIRE(!sc || a.scale_group > 0, "cgemm: scale_group must be positive");
   REQUIRE(epi == EPI_STORE || (a.out0 && a.P0), "cgemm: green epilogue needs P0/out0");
+  REQUIRE(a.tile >= 0 && a.tile < GEMM_TILE_COUNT, "cgemm: unknown tile shape %d", a.tile);
+  REQUIRE(epi != EPI_GREEN || gemm_tile_bn(a.tile) == 64, "cgemm: the Green epilogue needs a BN = 64 tile");
   if (epi == EPI_STORE) {
     if (opa == OP_N) return sc ? dispatch_tile<T, OP_N, EPI_STORE, true>(a, s) : dispatch_tile<T, OP_N, EPI_STORE, false>(a, s);
-    if (opa == OP_T) return dispatch_tile<T, OP_T, EPI_STORE, false>(a, s);
+    if (opa == OP_T) return dispatch_default<T, OP_T, EPI_STORE, false>(a, s);
     if (opa == OP_C) return dispatch_tile<T, OP_C, EPI_STORE, false>(a, s);
-    if (opa == OP_NR) return dispatch_tile<T, OP_NR, EPI_STORE, false>(a, s);
+    if (opa == OP_NR) return dispatch_default<T, OP_NR, EPI_STORE, false>(a, s);
   } else {
-    if (opa == OP_N) return dispatch_tile<T, OP_N, EPI_GREEN, false>(a, s);
+    if (opa == OP_N) return dispatch_green<T>(a, s);
   }
   afqmc_set_error("cgemm: unsupported op/epilogue combination (%d, %d)", opa, epi);
   return -1;

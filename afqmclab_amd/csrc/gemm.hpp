@@ -33,7 +33,13 @@ struct GemmArgs {
   // tile order within an XCD: 0 = M tiles fastest (B panels shared), 1 = N tiles fastest (A panels
   // shared; for a large shared A against many narrow B panels)
   int n_fastest;
+  // output tile shape (GEMM_TILE_*); the Green epilogue supports the BN = 64 shapes only
+  int tile;
 };
+
+// Output tile shapes of the MFMA GEMM (4 waves each): the autotuner picks one per call site.
+enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_64x32 = 2, GEMM_TILE_32x32 = 3, GEMM_TILE_COUNT = 4 };
+inline int gemm_tile_bn(int tile) { return (tile == GEMM_TILE_64x32 || tile == GEMM_TILE_32x32) ? 32 : 64; }
 
 template <typename T>
 int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s);

@@ -105,6 +105,16 @@ int launch_reset_measure(DevScalars* sc, zd* site_acc, int L, hipStream_t s) {
   return 0;
 }
 
+thread_local CaptureSession* g_capture = nullptr;
+
+void capture_note_pair(DevBuf<char>* a, DevBuf<char>* b) {
+  if (!g_capture) return;
+  for (auto& q : g_capture->pairs)
+    if (q.first == a || q.first == b) return;
+  g_capture->pairs.push_back({a, b});
+  g_capture->first.push_back(a->p);
+}
+
 extern "C" {
 
 const char* afqmc_last_error(void) { return g_err; }
@@ -178,6 +188,144 @@ int afqmc_ctx_get_timing(afqmc_ctx ctx, double* total_ms, long long* count, int 
   int rc = resolve_events(ctx);
   if (rc) return rc;
   for (int r = 0; r < n && r < REG_COUNT; ++r) { total_ms[r] = ctx->total_ms[r]; count[r] = ctx->count[r]; }
+  return 0;
+}
+
+int afqmc_capture_begin(afqmc_ctx ctx) { return afqmc_capture_begin_group(ctx, 0, nullptr); }
+
+int afqmc_capture_begin_group(afqmc_ctx origin, int n, const afqmc_ctx* members) {
+  REQUIRE(origin && n >= 0 && (n == 0 || members), "invalid argument");
+  REQUIRE(!g_capture, "a graph capture is already open on this thread");
+  for (int i = 0; i < n; ++i) {
+    REQUIRE(members[i] && members[i]->device == origin->device, "group members must be contexts on the origin's device");
+    for (int j = 0; j < i; ++j) REQUIRE(members[j] != members[i], "group members must be distinct contexts");
+    REQUIRE(members[i] != origin, "the origin context is not a member");
+  }
+  ScopedDevice sd(origin);
+  HIP_TRY(hipStreamSynchronize(origin->stream));
+  for (int i = 0; i < n; ++i) HIP_TRY(hipStreamSynchronize(members[i]->stream));
+  HIP_TRY(hipStreamBeginCapture(origin->stream, hipStreamCaptureModeRelaxed));
+  auto* cs = new CaptureSession();
+  cs->origin = origin;
+  if (n > 0) {
+    // members join the capture by waiting on an event recorded in it
+    hipEvent_t fork = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(fork, origin->stream);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+      e = hipStreamWaitEvent(members[i]->stream, fork, 0);
+      if (e == hipSuccess) cs->members.push_back(members[i]);
+    }
+    if (fork) hipEventDestroy(fork);
+    if (e != hipSuccess) {
+      hipGraph_t g = nullptr;
+      hipStreamEndCapture(origin->stream, &g);
+      if (g) hipGraphDestroy(g);
+      delete cs;
+      afqmc_set_error("afqmc_capture_begin_group: %s", hipGetErrorString(e));
+      return -2;
+    }
+  }
+  g_capture = cs;
+  return 0;
+}
+
+int afqmc_capture_end(afqmc_ctx origin, afqmc_graph* out) {
+  REQUIRE(origin && out, "null argument");
+  REQUIRE(g_capture && g_capture->origin == origin, "no graph capture is open for this context");
+  CaptureSession* cs = g_capture;
+  g_capture = nullptr;
+  ScopedDevice sd(origin);
+  hipError_t e = hipSuccess;
+  std::vector<hipEvent_t> joins;
+  for (afqmc_ctx m : cs->members) {
+    hipEvent_t j = nullptr;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(j, m->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(origin->stream, j, 0);
+    if (j) joins.push_back(j);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(origin->stream, &g);
+  for (hipEvent_t j : joins) hipEventDestroy(j);
+  // the captured work has not run: give the swapped buffer pairs back their roles
+  std::vector<GraphPair> pairs;
+  for (size_t i = 0; i < cs->pairs.size(); ++i) {
+    auto& q = cs->pairs[i];
+    const int swapped = q.first->p != cs->first[i];
+    if (swapped) swap(*q.first, *q.second);
+    pairs.push_back({q.first, q.second, cs->first[i], swapped});
+  }
+  std::vector<afqmc_ctx> members = cs->members;
+  delete cs;
+  if (e != hipSuccess || e2 != hipSuccess || !g) {
+    if (g) hipGraphDestroy(g);
+    afqmc_set_error("afqmc_capture_end: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    return -2;
+  }
+  auto* gr = new afqmc_graph_s();
+  gr->graph = g;
+  gr->origin = origin;
+  gr->members = std::move(members);
+  gr->pairs = std::move(pairs);
+  e = hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&gr->ev, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    if (gr->exec) hipGraphExecDestroy(gr->exec);
+    hipGraphDestroy(g);
+    delete gr;
+    afqmc_set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+    return -2;
+  }
+  *out = gr;
+  return 0;
+}
+
+int afqmc_graph_launch(afqmc_graph g) {
+  REQUIRE(g && g->exec, "null graph");
+  REQUIRE(!capturing(), "afqmc_graph_launch cannot be recorded into a graph");
+  ScopedDevice sd(g->origin);
+  hipStream_t s = g->origin->stream;
+  // order after work already queued on the member streams
+  for (afqmc_ctx m : g->members) {
+    HIP_TRY(hipEventRecord(g->ev, m->stream));
+    HIP_TRY(hipStreamWaitEvent(s, g->ev, 0));
+  }
+  // the graph reads each pair's data from the buffer it had at capture time: if eager calls
+  // since then left the data in the other buffer, move it back first
+  for (auto& q : g->pairs) {
+    if (q.a->p != q.start) {
+      HIP_TRY(hipMemcpyAsync(q.start, q.a->p, q.a->n, hipMemcpyDeviceToDevice, s));
+      swap(*q.a, *q.b);
+    }
+  }
+  HIP_TRY(hipGraphLaunch(g->exec, s));
+  for (auto& q : g->pairs)
+    if (q.swapped) swap(*q.a, *q.b);
+  // later work on the member streams follows the graph
+  if (!g->members.empty()) {
+    HIP_TRY(hipEventRecord(g->ev, s));
+    for (afqmc_ctx m : g->members) HIP_TRY(hipStreamWaitEvent(m->stream, g->ev, 0));
+  }
+  return 0;
+}
+
+int afqmc_graph_node_count(afqmc_graph g) {
+  if (!g || !g->graph) return -1;
+  size_t n = 0;
+  if (hipGraphGetNodes(g->graph, nullptr, &n) != hipSuccess) return -1;
+  return (int)n;
+}
+
+int afqmc_graph_destroy(afqmc_graph g) {
+  if (!g) return 0;
+  ScopedDevice sd(g->origin);
+  hipStreamSynchronize(g->origin->stream);
+  for (afqmc_ctx m : g->members) hipStreamSynchronize(m->stream);
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  if (g->ev) hipEventDestroy(g->ev);
+  delete g;
   return 0;
 }
 

@@ -24,13 +24,46 @@ struct afqmc_ctx_s {
   long long count[REG_COUNT] = {0};
 };
 
+template <typename T> struct DevBuf;
+
+// Stream capture into a HIP graph (afqmc_capture_begin / afqmc_capture_end).  While a capture is
+// open on this thread, entry points that only enqueue work skip their host synchronisation and
+// timing events, and walker sets record the buffer pairs whose roles the captured work swaps:
+// the graph replays those swaps on the host at every launch.
+struct CaptureSession {
+  afqmc_ctx origin = nullptr;
+  std::vector<afqmc_ctx> members;
+  std::vector<std::pair<DevBuf<char>*, DevBuf<char>*>> pairs;
+  std::vector<void*> first;  // pairs[i].first->p when the pair was noted
+};
+extern thread_local CaptureSession* g_capture;
+inline bool capturing() { return g_capture != nullptr; }
+inline hipError_t sync_unless_capturing(hipStream_t s) { return g_capture ? hipSuccess : hipStreamSynchronize(s); }
+void capture_note_pair(DevBuf<char>* a, DevBuf<char>* b);
+#define NOT_CAPTURING() REQUIRE(!capturing(), "%s cannot be recorded into a graph (it synchronises)", __func__)
+
+struct GraphPair {
+  DevBuf<char>* a;
+  DevBuf<char>* b;
+  void* start;  // a->p when the capture began: the buffer the graph reads as `a`
+  int swapped;  // the captured work leaves the pair with roles exchanged
+};
+struct afqmc_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  afqmc_ctx origin = nullptr;
+  std::vector<afqmc_ctx> members;
+  std::vector<GraphPair> pairs;
+  hipEvent_t ev = nullptr;
+};
+
 // RAII bracket of one tagged region on the context stream
 struct RegionTimer {
   afqmc_ctx ctx;
   int reg;
   hipEvent_t a = nullptr, b = nullptr;
   RegionTimer(afqmc_ctx c, int r) : ctx(c), reg(r) {
-    if (ctx && ctx->timing) {
+    if (ctx && ctx->timing && !capturing()) {
       hipEventCreate(&a);
       hipEventCreate(&b);
       hipEventRecord(a, ctx->stream);

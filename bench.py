@@ -158,6 +158,90 @@ def run_phaseless(ctx, mol, precision, nw, steps, warmup, seed, meas=5):
     return dict(elapsed=elapsed, walker_steps=wsteps, timing=timing, energy=E)
 
 
+def hetero_instances(n_each=6, seed=20241):
+    """24 Hubbard instances: 6x6, 8x8, 10x10, 12x12 (n_each each), filling 0.75 / 0.875 / 1.0 with
+    N rounded to the nearest even number (Nup = Ndn), U uniform in [2, 8], t' = -0.2 on odd ones."""
+    from afqmclab_amd.model import HubbardModel, spin_block_trial
+    rng = np.random.default_rng(seed)
+    out = []
+    fills = [0.75, 0.875, 1.0]
+    for li, n in enumerate([6, 8, 10, 12]):
+        for j in range(n_each):
+            idx = len(out)
+            Ls = n * n
+            ntot = int(round(fills[j % 3] * Ls / 2.0)) * 2
+            nup = ntot // 2
+            U = float(rng.uniform(2.0, 8.0))
+            t2 = -0.2 if idx % 2 == 1 else 0.0
+            model = HubbardModel.square(n, n, ntot, U, 1.0, t2)
+            out.append((f"{n}x{n} N={ntot} U={U:.2f} t'={t2}", model, spin_block_trial(model, nup, nup), nup))
+    return out
+
+
+def run_hetero(instances, nw, steps, seed, tuned_graph):
+    """All instances for `steps` steps (QR every 10, comb every 40).  tuned_graph=False: default
+    tiles, per-instance launches on one stream.  True: autotuned tiles, each instance on its own
+    stream, every 10-step block of all instances one HIP-graph launch."""
+    from afqmclab_amd.cp import Context, CPMethod, CPPropagator
+    origin = Context(0)
+    ctxs = [Context(0) for _ in instances] if tuned_graph else [origin] * len(instances)
+    sets = []
+    tiles = []
+    for i, ((name, model, phi, nup), c) in enumerate(zip(instances, ctxs)):
+        m = CPMethod(dt=0.01, walkerSizePerThread=nw, decompType="densitySpin", ET=-1.0)
+        prop = CPPropagator(c, model, m, phi, nup=nup, spin_blocks=True, precision=1, max_walkers=nw)
+        prop.set_seed(seed + i, 0)
+        prop.set_ET(-1.0)
+        ws = prop.walkers(nw)
+        ws.set_from_trial(phi)
+        prop.stabilize(ws)
+        prop.pop_control(ws)
+        prop.onebody(ws, 1)
+        sets.append((prop, ws))
+    if tuned_graph:
+        for prop, ws in sets:
+            t, _ = prop.autotune(ws, reps=3)
+            tiles.append([CPPropagator.TILES[k] for k in t])
+    for c in set(ctxs):
+        c.synchronize()
+
+    def block(with_comb):
+        for prop, ws in sets:
+            for _ in range(10):
+                prop.step(ws)
+            prop.stabilize(ws)
+            if with_comb:
+                prop.pop_control(ws)
+
+    graphs = {}
+    if tuned_graph:
+        for with_comb in (False, True):
+            origin.capture_begin(members=ctxs)
+            block(with_comb)
+            graphs[with_comb] = origin.capture_end()
+    nblocks = steps // 10
+    t0 = time.perf_counter()
+    for b in range(nblocks):
+        with_comb = (b + 1) % 4 == 0
+        if tuned_graph:
+            graphs[with_comb].launch()
+        else:
+            block(with_comb)
+    origin.synchronize()
+    for c in set(ctxs):
+        c.synchronize()
+    elapsed = time.perf_counter() - t0
+    logo = [prop.log_overlap(ws) for prop, ws in sets]
+    nodes = graphs[False].nodes() if tuned_graph else None
+    for g in graphs.values():
+        g.close()
+    for prop, ws in sets:
+        ws.close()
+        prop.close()
+    wsteps = len(instances) * nw * nblocks * 10
+    return dict(elapsed=elapsed, walker_steps_per_s=wsteps / elapsed, logo=logo, tiles=tiles, graph_nodes=nodes)
+
+
 def cpu_baseline(model, phi, nw_cpu, budget_s, sched, seed):
     """Oracle (dense reference representation) on a bounded sample of the same workload."""
     from oracle.blocks import SD
@@ -199,6 +283,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ph-steps", type=int, default=60, help="phaseless leg steps (0 = skip)")
     ap.add_argument("--ph-walkers", type=int, default=1024)
+    ap.add_argument("--hetero-steps", type=int, default=200, help="heterogeneous batch leg steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (reference kernel for HBM traffic counters)")
     args = ap.parse_args()
@@ -315,6 +400,24 @@ def main():
                             "steps": args.ph_steps, "paths": ph}
         if "fp64" in ph and "mixed" in ph:
             out["phaseless"]["mixed_vs_fp64"] = ph["mixed"]["walker_steps_per_s"] / ph["fp64"]["walker_steps_per_s"]
+    if args.hetero_steps > 0:
+        inst = hetero_instances()
+        log(f"[bench] heterogeneous batch: {len(inst)} Hubbard instances x 256 walkers, {args.hetero_steps} steps")
+        eager = run_hetero(inst, 256, args.hetero_steps, args.seed, False)
+        log(f"[bench]   per-instance launches, default tiles: {eager['walker_steps_per_s']:.0f} walker-steps/s")
+        graph = run_hetero(inst, 256, args.hetero_steps, args.seed, True)
+        log(f"[bench]   one graph per 10-step block, autotuned tiles: {graph['walker_steps_per_s']:.0f} walker-steps/s")
+        same = all(np.allclose(a, b, rtol=1e-12, atol=0) for a, b in zip(eager["logo"], graph["logo"]))
+        out["batched_hetero"] = {
+            "workload": "24 Hubbard instances 6x6..12x12 (6 each), filling 0.75-1.0, U in [2,8], t'=-0.2 on odd "
+                        "instances, 256 walkers each (6144), dt=0.01, mixed precision, QR every 10, comb every 40",
+            "steps": args.hetero_steps,
+            "per_instance_launches": {"walker_steps_per_s": eager["walker_steps_per_s"], "seconds": eager["elapsed"]},
+            "graph_autotuned": {"walker_steps_per_s": graph["walker_steps_per_s"], "seconds": graph["elapsed"],
+                                "graph_nodes_per_block": graph["graph_nodes"], "tiles": graph["tiles"]},
+            "speedup": graph["walker_steps_per_s"] / eager["walker_steps_per_s"],
+            "same_log_overlaps": bool(same),
+        }
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle, {args.cpu_walkers} walkers, ~{args.cpu_seconds}s")
         ws_s, nst = cpu_baseline(model, phi, args.cpu_walkers, args.cpu_seconds, sched, args.seed)

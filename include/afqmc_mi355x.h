@@ -25,6 +25,7 @@ typedef struct afqmc_ctx_s* afqmc_ctx;
 typedef struct afqmc_cp_s* afqmc_cp;
 typedef struct afqmc_walkers_s* afqmc_walkers;
 typedef struct afqmc_ph_s* afqmc_ph;
+typedef struct afqmc_graph_s* afqmc_graph;
 
 /* ------------------------------------------------------------------ context */
 const char* afqmc_last_error(void);
@@ -41,6 +42,21 @@ int afqmc_ctx_synchronize(afqmc_ctx ctx);
  * set_timing resets the totals; get_timing synchronises and returns n totals (ms) and counts. */
 int afqmc_ctx_set_timing(afqmc_ctx ctx, int enable);
 int afqmc_ctx_get_timing(afqmc_ctx ctx, double* total_ms, long long* count, int n);
+
+/* HIP-graph capture of enqueue-only calls (no reference counterpart: the reference launches
+ * nothing).  Between begin and end, the calls made on this thread are recorded instead of run:
+ * afqmc_cp_step (without records), afqmc_cp_stabilize, afqmc_cp_pop_control (without outputs),
+ * afqmc_cp_measure, afqmc_cp_onebody, afqmc_cp_reset_measure, afqmc_cp_adjust_ET.  Calls that
+ * return data to the host fail while a capture is open.  begin_group also records work issued on
+ * the member contexts (one stream each; e.g. independent lattices), which become parallel
+ * branches of one graph.  afqmc_graph_launch enqueues the whole graph on the origin's stream
+ * (asynchronous).  Walker sets and propagators used in a graph must outlive it. */
+int afqmc_capture_begin(afqmc_ctx ctx);
+int afqmc_capture_begin_group(afqmc_ctx origin, int n, const afqmc_ctx* members);
+int afqmc_capture_end(afqmc_ctx origin, afqmc_graph* out);
+int afqmc_graph_launch(afqmc_graph g);
+int afqmc_graph_node_count(afqmc_graph g);
+int afqmc_graph_destroy(afqmc_graph g);
 
 /* ------------------------------------------------------------------ CP (Hubbard) */
 /* Model + method description.  Mirrors HubbardSOC (HubbardSOC.cpp:39-59), the
@@ -101,6 +117,15 @@ int afqmc_cp_set_uniforms(afqmc_cp cp, const double* u, int64_t n); /* n = 0: ba
 int afqmc_cp_skip_uniforms(afqmc_cp cp, uint64_t n);
 int afqmc_cp_set_ET(afqmc_cp cp, double ET);
 int afqmc_cp_get_ET(afqmc_cp cp, double* ET);
+
+/* GEMM tile shapes of the step (no reference counterpart): tiles[0] the one-body propagator,
+ * tiles[1] the overlap, tiles[2] the Green's function GEMM; 0 = 64x64, 1 = 32x64, 2 = 64x32,
+ * 3 = 32x32 output tiles (the Green GEMM takes 0 or 1).  autotune times every shape on this
+ * walker set's own GEMMs (reps launches each, walker state unchanged), keeps the fastest and
+ * returns the choices in tiles (3) and the times in ms (3 x 4, ms per launch, -1 = not a
+ * candidate); either output may be NULL. */
+int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles);
+int afqmc_cp_autotune(afqmc_cp cp, afqmc_walkers w, int reps, int* tiles, double* ms);
 
 /* SDSDOperation::returnLogOverlap (SDSDOperation.cpp:65-72) for every walker: logO nw complex */
 int afqmc_cp_log_overlap(afqmc_cp cp, afqmc_walkers w, double* logO);
