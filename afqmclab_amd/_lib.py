@@ -88,6 +88,8 @@ SIGNATURES = {
     "afqmc_cp_site_measure": (C.c_int, [_vp, _dp]),
     "afqmc_cp_reset_measure": (C.c_int, [_vp]),
     "afqmc_cp_adjust_ET": (C.c_int, [_vp]),
+    "afqmc_cp_fixed_field_energy": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_uint64, _dp]),
+    "afqmc_cp_fixed_field_adjoint": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_uint64, _dp, _dp, _dp, _dp]),
     "afqmc_ph_create": (C.c_int, [_vp, C.POINTER(PhDesc), C.POINTER(_vp)]),
     "afqmc_ph_destroy": (C.c_int, [_vp]),
     "afqmc_ph_set_onebody": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int]),

@@ -165,6 +165,29 @@ int afqmc_cp_reset_measure(afqmc_cp cp);
 /* adjustETAndResetMeasurement (afqmcConstraintPathMeasure.cpp:35-45): ET = Re(HNum/den) on device, then reset */
 int afqmc_cp_adjust_ET(afqmc_cp cp);
 
+/* Fixed-field propagation and its adjoint (no reference counterpart: the reference has no
+ * gradients; this is the differentiable segment of a constraint-path run, for sensitivities of the
+ * mixed-estimator energy to the Hamiltonian).  FP64 spin-block propagators with the densitySpin
+ * decomposition and mu = h = 0.  Steps s = 0 .. nsteps-1 apply, per spin block (sigma = +1 up,
+ * -1 down), W <- exp(-dt K) diag(exp(sigma gamma_i x) constDiag_i) W with the field
+ * x_{s,w,i} = (u < 1/2 ? -1 : +1), u the uniform number (s*nw + w)*L + i of the counter-based
+ * stream seeded with field_seed; no force bias, phase projection or comb; CholeskyQR every mgs
+ * steps with logw += log det R.  The energy is the mixed estimator of the final walkers,
+ * E = sum_w O_w E_L,w / sum_w O_w (O_w = exp(logw) times the overlaps with the trial, E_L the
+ * local energy of HubbardSOCMeasureFixedSDSD.cpp:178-218), one complex (re, im).  Both calls start
+ * from the walker set's current state and restore it.
+ * afqmc_cp_fixed_field_adjoint: forward with a checkpoint every ckpt steps (a multiple of mgs),
+ * then the reverse sweep, recomputing each segment from its checkpoint.  Outputs:
+ *   bbar (2 x L x L complex): dE/dB_ij for B = exp(-dt K_b) of each spin block, summed over steps;
+ *   mbar (2 x L x L complex): the explicit dE/dK_ij of each block's kinetic term;
+ *   ubar (L complex): dE/dU_i, through the HS factors and the interaction term.
+ * dE/dK_b follows from bbar by the derivative of the matrix exponential (done by the caller,
+ * which holds K's eigendecomposition) plus mbar. */
+int afqmc_cp_fixed_field_energy(afqmc_cp cp, afqmc_walkers w, int nsteps, int mgs, uint64_t field_seed,
+                                double* energy);
+int afqmc_cp_fixed_field_adjoint(afqmc_cp cp, afqmc_walkers w, int nsteps, int mgs, int ckpt, uint64_t field_seed,
+                                 double* energy, double* bbar, double* mbar, double* ubar);
+
 /* ------------------------------------------------------------------ phaseless (real materials) */
 /* Model, trial and method of the phaseless driver: RealMaterialMolecule (realMaterialMolecule.h:16-40,
  * t, choleskyVecs, choleskyBg), the SD2s trial, the half-rotated vectors of
