@@ -60,7 +60,7 @@ def flops_per_walker_step(Ls, nup, ndn, mgs, pop, meas):
     return 8.0 * (step + qr + comb + measure), 8.0 * sum(Ls * Ls * n for n in (nup, ndn))
 
 
-def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched):
+def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, keep_walkers=False, block=50):
     from afqmclab_amd.cp import CPMethod, CPPropagator
     m = CPMethod(dt=sched["dt"], walkerSizePerThread=nw, decompType="densitySpin", forceType="dynamicForce",
                  forceCap=1.5, mgsStep=sched["mgs"], popControlStep=sched["pop"], ET=-10.0)
@@ -102,9 +102,14 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched):
     ws0 = prop.walker_steps()
     ctx.set_timing(True)
     ctx.synchronize()
+    blocks = []  # (HNum, den) of consecutive measurement blocks after thermalisation
     t0 = time.perf_counter()
     for i in range(warmup, warmup + steps):
         one_step(i)
+        if i >= sched["thermal"] and (i - sched["thermal"] + 1) % block == 0:
+            e = prop.energy()
+            blocks.append((e["HNum"], e["den"]))
+            prop.reset_measure()
         if (i - warmup + 1) % 100 == 0:
             log(f"[bench]   step {i - warmup + 1}/{steps}")
     ctx.synchronize()
@@ -113,10 +118,38 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched):
     ctx.set_timing(False)
     wsteps = prop.walker_steps() - ws0
     e = prop.energy()
-    E = (e["HNum"] / e["den"]) if abs(e["den"]) > 0 else complex("nan")
+    if abs(e["den"]) > 0:
+        blocks.append((e["HNum"], e["den"]))
+    H = sum(b[0] for b in blocks)
+    D = sum(b[1] for b in blocks)
+    E = H / D if abs(D) > 0 else complex("nan")
+    eb = np.array([(h / d).real for h, d in blocks if abs(d) > 0])
+    err = float(eb.std(ddof=1) / np.sqrt(len(eb))) if len(eb) > 1 else float("nan")
+    walkers = ws.download() if keep_walkers else None
     ws.close()
     prop.close()
-    return dict(elapsed=elapsed, walker_steps=wsteps, timing=timing, energy=E, den=e["den"])
+    return dict(elapsed=elapsed, walker_steps=wsteps, timing=timing, energy=E, err=err, nblocks=len(eb),
+                walkers=walkers)
+
+
+def estimator_check(ctx, model, phi, nup, nw, walkers):
+    """The mixed estimator of one and the same walker population evaluated by the FP64 and the
+    mixed-precision paths: the precision error of the estimator itself, free of sampling noise."""
+    from afqmclab_amd.cp import CPMethod, CPPropagator
+    out = []
+    for precision in (0, 1):
+        m = CPMethod(dt=0.005, walkerSizePerThread=nw, decompType="densitySpin")
+        prop = CPPropagator(ctx, model, m, phi, nup=nup, spin_blocks=True, precision=precision, max_walkers=nw)
+        ws = prop.walkers(nw)
+        wf, lw, al = walkers
+        ws.upload(wf, lw, al)
+        prop.reset_measure()
+        prop.measure(ws)
+        e = prop.energy()
+        out.append(e["HNum"] / e["den"])
+        ws.close()
+        prop.close()
+    return out
 
 
 def run_phaseless(ctx, mol, precision, nw, steps, warmup, seed, meas=5):
@@ -242,6 +275,43 @@ def run_hetero(instances, nw, steps, seed, tuned_graph):
     return dict(elapsed=elapsed, walker_steps_per_s=wsteps / elapsed, logo=logo, tiles=tiles, graph_nodes=nodes)
 
 
+def run_ed_check(ctx):
+    """4x4 periodic Hubbard, U = 4, 5 + 5, free-electron trial, 128 walkers, 40 + 200 steps: the
+    block-averaged CP energy next to the exact ground-state energy of the same sector."""
+    from afqmclab_amd.cp import AfqmcConstraintPath, CPMethod
+    from afqmclab_amd.model import HubbardModel, free_electron_trial
+    with open(os.path.join(ROOT, "tests", "golden", "hubbard4x4_u4_n5u5d_ed.json")) as f:
+        ed = json.load(f)
+    model = HubbardModel.square(4, 4, 10, 4.0)
+    m = CPMethod(dt=0.01, thermalSize=40, writeNumber=20, measureNumberPerWrite=2, measureSkipStep=5,
+                 walkerSizePerThread=128, decompType="densitySpin", mgsStep=10, popControlStep=40, ETAdjustStep=10,
+                 ETAdjustMaxSize=40, seed=8501)
+    t0 = time.perf_counter()
+    drv = AfqmcConstraintPath(ctx, model, m, free_electron_trial(model), spin_blocks=False, precision=0)
+    drv.run()
+    e = np.array([(h / d).real for h, d, *_ in drv.energies])
+    return {"E": float(e.mean()), "err": float(e.std(ddof=1) / np.sqrt(len(e))), "E_exact": ed["E0"],
+            "seconds": time.perf_counter() - t0, "config": "4x4 U=4 5+5, 128 walkers, 40 + 200 steps, FP64"}
+
+
+def run_gradient_leg(ctx):
+    """8x8 periodic, t = 1, t' = -0.25, U = 6, 28 + 28, 512 walkers, 400 fixed-field steps, QR every
+    10, checkpoints every 50: forward and adjoint times, and dE/dU against central differences."""
+    from afqmclab_amd.adjoint import FixedFieldHubbard
+    eng = FixedFieldHubbard(ctx, 8, 8, 28, 28, t1=1.0, t2=-0.25, U=6.0, dt=0.01, nw=512, nsteps=400, mgs=10,
+                            ckpt=50, field_seed=2025)
+    t0 = time.perf_counter()
+    E = eng.energy(eng.t0, eng.U0)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, gt, gU = eng.energy_and_grad(eng.t0, eng.U0)
+    t_adj = time.perf_counter() - t0
+    h = 1e-5
+    fdU = (eng.energy(eng.t0, eng.U0 + h) - eng.energy(eng.t0, eng.U0 - h)).real / (2 * h)
+    return {"E": E.real, "forward_s": t_fwd, "forward_and_adjoint_s": t_adj, "n_bond_gradients": int(gt.size),
+            "dE_dU": gU, "dE_dU_central_difference": fdU, "rel_diff": abs(gU - fdU) / max(1.0, abs(fdU))}
+
+
 def cpu_baseline(model, phi, nw_cpu, budget_s, sched, seed):
     """Oracle (dense reference representation) on a bounded sample of the same workload."""
     from oracle.blocks import SD
@@ -284,6 +354,7 @@ def main():
     ap.add_argument("--ph-steps", type=int, default=60, help="phaseless leg steps (0 = skip)")
     ap.add_argument("--ph-walkers", type=int, default=1024)
     ap.add_argument("--hetero-steps", type=int, default=200, help="heterogeneous batch leg steps (0 = skip)")
+    ap.add_argument("--extras", type=int, default=1, help="run the 4x4-vs-ED and 8x8 gradient legs (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (reference kernel for HBM traffic counters)")
     args = ap.parse_args()
@@ -315,7 +386,7 @@ def main():
         precision = 1 if path == "mixed" else 0
         log(f"[bench] path {path}: {args.walkers} walkers, {args.warmup} warmup + {args.steps} timed steps")
         results[path] = run_path(ctx, model, phi, nup, precision, args.walkers, args.steps, args.warmup, args.seed,
-                                 sched)
+                                 sched, keep_walkers=(path == "fp64"))
         r = results[path]
         log(f"[bench]   {path}: {r['elapsed']:.3f}s, {r['walker_steps'] / r['elapsed']:.1f} walker-steps/s, "
             f"E={r['energy']:.8f}")
@@ -369,6 +440,8 @@ def main():
             "walker_steps_per_s": rr["walker_steps"] / rr["elapsed"],
             "ms_per_step": rr["elapsed"] / args.steps * 1e3,
             "energy_per_site": (rr["energy"] / Ls).real,
+            "energy_per_site_err": rr["err"] / Ls,
+            "energy_blocks": rr["nblocks"],
             "tflops_overall": fl_ws * rr["walker_steps"] / rr["elapsed"] / 1e12,
             "kernel_ms": {k: round(v[0], 3) for k, v in rr["timing"].items() if v[1]},
         }
@@ -376,6 +449,9 @@ def main():
         f, mx = results["fp64"], results["mixed"]
         out["mixed_vs_fp64"] = (mx["walker_steps"] / mx["elapsed"]) / (f["walker_steps"] / f["elapsed"])
         out["dE_per_site_mixed_vs_fp64"] = abs((mx["energy"] - f["energy"]) / Ls)
+        out["dE_per_site_stat_err"] = float(np.hypot(f["err"], mx["err"]) / Ls)
+        e64, e32 = estimator_check(ctx, model, phi, nup, args.walkers, f["walkers"])
+        out["dE_per_site_estimator_same_walkers"] = abs((e32 - e64) / Ls)
     if args.ph_steps > 0:
         from afqmclab_amd.phaseless import synthetic_molecule
         t0 = time.time()
@@ -418,6 +494,12 @@ def main():
             "speedup": graph["walker_steps_per_s"] / eager["walker_steps_per_s"],
             "same_log_overlaps": bool(same),
         }
+    if args.extras:
+        out["hubbard4x4_vs_exact"] = run_ed_check(ctx)
+        log(f"[bench] 4x4 U=4 5+5: E = {out['hubbard4x4_vs_exact']['E']:.4f} +- "
+            f"{out['hubbard4x4_vs_exact']['err']:.4f}, exact {out['hubbard4x4_vs_exact']['E_exact']:.4f}")
+        out["gradient_8x8"] = run_gradient_leg(ctx)
+        log(f"[bench] 8x8 fixed-field gradient: {out['gradient_8x8']}")
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle, {args.cpu_walkers} walkers, ~{args.cpu_seconds}s")
         ws_s, nst = cpu_baseline(model, phi, args.cpu_walkers, args.cpu_seconds, sched, args.seed)
