@@ -245,7 +245,7 @@ class CPPropagator:
         return WalkerSet(self, nw)
 
     # ---- GEMM tile shapes -------------------------------------------------
-    TILES = ["64x64", "32x64", "64x32", "32x32"]
+    TILES = ["64x64", "32x64", "64x32", "32x32", "128x64", "64x64k32"]
 
     def set_tiles(self, propagate=0, overlap=0, green=0):
         t = np.array([propagate, overlap, green], dtype=np.int32)
@@ -253,11 +253,12 @@ class CPPropagator:
 
     def autotune(self, w, reps=5):
         """Time every tile shape on this walker set's propagate / overlap / Green GEMMs, keep the
-        fastest; returns (tiles, ms[3][4])."""
+        fastest; returns (tiles, ms[3][n_tiles])."""
+        n = len(self.TILES)
         t = np.zeros(3, dtype=np.int32)
-        ms = np.zeros(3 * 4)
+        ms = np.zeros(3 * n)
         check(lib().afqmc_cp_autotune(self.h, w.h, int(reps), iptr(t), dptr(ms)))
-        return t, ms.reshape(3, 4)
+        return t, ms.reshape(3, n)
 
     # ---- batched drop-ins -------------------------------------------------
     def log_overlap(self, w):

@@ -152,7 +152,10 @@ int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out) {
 }
 
 // CholeskyQR2 in FP64 of each block: W <- W R^{-1}, logw += sum log R_kk (both passes)
-int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tmp) {
+// CholeskyQR in FP64 of each block: S = W^H W = R^H R, W <- W R^{-1}, logw += sum log R_kk.  A
+// second pass (CholeskyQR2) runs only for the walkers whose Q^H Q is still farther than `tol` from
+// the identity after the first; the others keep their first-pass Q.
+int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tmp, double tol) {
   hipStream_t s = cp->ctx->stream;
   int rc;
   for (int pass = 0; pass < 2; ++pass) {
@@ -165,40 +168,50 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
       a.C = cp->qrS[b].p; a.ldc = Nb; a.strideC = (long long)Nb * Nb;
       a.alive = w->alive.p;
       if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
+      const int* mask = w->alive.p;
+      if (pass == 1) {
+        if ((rc = launch_ortho_check(w->nw, Nb, (const zd*)cp->qrS[b].p, w->alive.p, tol, cp->qr_need.p,
+                                     cp->qr_skip.p, s)))
+          return rc;
+        mask = cp->qr_need.p;
+      }
       SmallDenseArgs c;
       memset(&c, 0, sizeof(c));
       c.N = Nb; c.batch = w->nw; c.in = cp->qrS[b].p; c.ld_in = Nb; c.stride_in = (long long)Nb * Nb;
       c.out = cp->qrS[b].p; c.ld_out = Nb; c.stride_out = c.stride_in;
-      c.logdet = cp->qrld[b].p; c.info = cp->qrinfo[b].p; c.nsub = Nb; c.alive = w->alive.p; c.alive_div = 1;
+      c.logdet = cp->qrld[b].p; c.info = cp->qrinfo[b].p; c.nsub = Nb; c.alive = mask; c.alive_div = 1;
       if ((rc = launch_chol_inverse<double>(c, s))) return rc;
       GemmArgs g = gemm_base<double>();
       g.M = Mb; g.N = Nb; g.K = Nb; g.batch = w->nw;
       g.A = wf[b].p; g.lda = Mb; g.strideA = w->wsize(b);
       g.B = cp->qrS[b].p; g.ldb = Nb; g.strideB = (long long)Nb * Nb;
       g.C = tmp[b].p; g.ldc = Mb; g.strideC = w->wsize(b);
-      g.alive = w->alive.p;
+      g.alive = mask;
       if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
+      if (pass == 1 &&
+          (rc = launch_select_alive(w->nw, w->wsize(b), sizeof(zd), cp->qr_skip.p, wf[b].p, tmp[b].p, s)))
+        return rc;
       std::swap(wf[b], tmp[b]);
-      if ((rc = launch_logw_add_real(w->nw, w->logw.p, w->alive.p, cp->qrld[b].p, nullptr, 0, 1.0, 0.0, s))) return rc;
+      if ((rc = launch_logw_add_real(w->nw, w->logw.p, mask, cp->qrld[b].p, nullptr, 0, 1.0, 0.0, s))) return rc;
     }
   }
   return 0;
 }
-
 int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
   RegionTimer t(cp->ctx, REG_QR);
   for (int b = 0; b < cp->nblk; ++b) {
     capture_note_pair(&w->wf[b], &w->tmp[b]);
     capture_note_pair(&cp->qr64[b], &cp->qr64b[b]);
   }
-  if (!cp->fp32) return qr_impl_f64(cp, w, w->wf, w->tmp);
+  if (!cp->fp32) return qr_impl_f64(cp, w, w->wf, w->tmp, 1e-13);
   // mixed path: QR in FP64 on a widened copy of the FP32 walkers
   hipStream_t s = cp->ctx->stream;
   int rc;
   for (int b = 0; b < cp->nblk; ++b) {
     if ((rc = launch_convert_f32_f64(w->wf[b].p, cp->qr64[b].p, w->wsize(b) * w->nw, s))) return rc;
   }
-  if ((rc = qr_impl_f64(cp, w, cp->qr64, cp->qr64b))) return rc;
+  // the result is rounded to FP32, so a first-pass Q orthonormal to 1e-8 is final
+  if ((rc = qr_impl_f64(cp, w, cp->qr64, cp->qr64b, 1e-8))) return rc;
   for (int b = 0; b < cp->nblk; ++b) {
     if ((rc = launch_convert_f64_f32(cp->qr64[b].p, w->wf[b].p, w->wsize(b) * w->nw, s))) return rc;
   }
@@ -429,6 +442,8 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
   if (!rc) rc = cp->prefix.alloc(nw);
   if (!rc) rc = cp->table.alloc(nw);
   if (!rc) rc = cp->comb_num.alloc(nw);
+  if (!rc) rc = cp->qr_need.alloc(nw);
+  if (!rc) rc = cp->qr_skip.alloc(nw);
   if (!rc) rc = cp->comb_prob.alloc(nw);
   if (!rc) rc = cp->weight.alloc(nw);
   if (!rc) rc = cp->per_walker.alloc((size_t)nw * 8);
@@ -919,7 +934,7 @@ int afqmc_cp_adjust_ET(afqmc_cp cp) {
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles) {
   REQUIRE(cp && tiles, "null argument");
   for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= 0 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
-  REQUIRE(gemm_tile_bn(tiles[2]) == 64, "the Green GEMM takes tile 0 or 1");
+  REQUIRE(gemm_tile_bn(tiles[2]) == 64, "the Green GEMM takes a 64-column tile (0, 1, 4 or 5)");
   for (int r = 0; r < 3; ++r) cp->tile[r] = tiles[r];
   return 0;
 }

@@ -441,6 +441,34 @@ __global__ void site_final_kernel(EnergyArgs a) {
   a.site_acc[e] = cadd(a.site_acc[e], s);
 }
 
+// need[w] = alive && max_ij |S_ij - delta_ij| > tol (S = Q^H Q of walker w), skip[w] = alive && !need
+__global__ void ortho_check_kernel(int N, const zd* S, const int* alive, double tol, int* need, int* skip) {
+  const int w = blockIdx.x;
+  __shared__ double sh[8];
+  const zd* s = S + (long long)w * N * N;
+  double m = 0.0;
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+    const int i = e % N, j = e / N;
+    const zd v = s[e];
+    const double d = fabs(v.re - (i == j ? 1.0 : 0.0)) + fabs(v.im);
+    m = (d > m || d != d) ? d : m;  // NaN counts as not orthonormal
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double x = __shfl_xor(m, o, 64);
+    m = (x > m || x != x) ? x : m;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) t = (sh[k] > t || sh[k] != sh[k]) ? sh[k] : t;
+    const int a = alive ? alive[w] : 1;
+    const int n = a && !(t <= tol);
+    need[w] = n;
+    skip[w] = a && !n;
+  }
+}
+
 __global__ void set_et_kernel(DevScalars* sc) {
   zd e = cdiv(sc->HNum, sc->den);
   sc->ET = e.re;
@@ -524,6 +552,14 @@ int launch_energy(const EnergyArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(site_partial_kernel, dim3((a.L + 63) / 64, SITE_CHUNKS), dim3(64), 0, s, a);
   HIP_LAUNCH_CHECK();
   hipLaunchKernelGGL(site_final_kernel, dim3((4 * a.L + 255) / 256), dim3(256), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, int* skip,
+                       hipStream_t s) {
+  if (nw <= 0) return 0;
+  hipLaunchKernelGGL(ortho_check_kernel, dim3(nw), dim3(256), 0, s, N, S, alive, tol, need, skip);
   HIP_LAUNCH_CHECK();
   return 0;
 }

@@ -284,6 +284,8 @@ int dispatch_tile(const GemmArgs& a, hipStream_t s) {
     case GEMM_TILE_32x64: return run<T, OPA, 32, 64, 16, 2, 2, EPI, SCALE>(a, s);
     case GEMM_TILE_64x32: return run<T, OPA, 64, 32, 16, 2, 2, EPI, SCALE>(a, s);
     case GEMM_TILE_32x32: return run<T, OPA, 32, 32, 16, 2, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_128x64: return run<T, OPA, 128, 64, 16, 2, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_64x64_K32: return run<T, OPA, 64, 64, 32, 2, 2, EPI, SCALE>(a, s);
     default: return run<T, OPA, 64, 64, 16, 2, 2, EPI, SCALE>(a, s);
   }
 }
@@ -297,6 +299,8 @@ int dispatch_default(const GemmArgs& a, hipStream_t s) {
 template <typename T>
 int dispatch_green(const GemmArgs& a, hipStream_t s) {
   if (a.tile == GEMM_TILE_32x64) return run<T, OP_N, 32, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
+  if (a.tile == GEMM_TILE_128x64) return run<T, OP_N, 128, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
+  if (a.tile == GEMM_TILE_64x64_K32) return run<T, OP_N, 64, 64, 32, 2, 2, EPI_GREEN, false>(a, s);
   return run<T, OP_N, 64, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
 }
 

@@ -38,7 +38,10 @@ struct GemmArgs {
 };
 
 // Output tile shapes of the MFMA GEMM (4 waves each): the autotuner picks one per call site.
-enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_64x32 = 2, GEMM_TILE_32x32 = 3, GEMM_TILE_COUNT = 4 };
+enum {
+  GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_64x32 = 2, GEMM_TILE_32x32 = 3, GEMM_TILE_128x64 = 4,
+  GEMM_TILE_64x64_K32 = 5, GEMM_TILE_COUNT = 6
+};
 inline int gemm_tile_bn(int tile) { return (tile == GEMM_TILE_64x32 || tile == GEMM_TILE_32x32) ? 32 : 64; }
 
 template <typename T>

@@ -74,6 +74,9 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, ke
     prop.pop_control(ws)
     prop.skip_uniforms(model.L)
     prop.onebody(ws, 1)
+    # GEMM tile shapes picked by the autotuner on these walkers' own shapes (state unchanged)
+    tiles, _ = prop.autotune(ws, reps=3)
+    tiles = [CPPropagator.TILES[t] for t in tiles]
     ctx.synchronize()
     idx = [0, 0]
     energies = []
@@ -129,7 +132,7 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, ke
     ws.close()
     prop.close()
     return dict(elapsed=elapsed, walker_steps=wsteps, timing=timing, energy=E, err=err, nblocks=len(eb),
-                walkers=walkers)
+                walkers=walkers, tiles=tiles)
 
 
 def estimator_check(ctx, model, phi, nup, nw, walkers):
@@ -418,7 +421,7 @@ def main():
                                f"{args.walkers} walkers, spin-block walkers 2x({Ls}x{nup})",
                    "walkers": args.walkers, "path": main_path, "qr_every": sched["mgs"], "comb_every": sched["pop"],
                    "measure_every": sched["meas"]},
-        "roofline": {"bound": "mfma", "kernel": "cgemm propagate (exp(-dt K) diag(HS) W)",
+        "roofline": {"bound": "mfma", "kernel": f"cgemm propagate (exp(-dt K) diag(HS) W), {r['tiles'][0]} tiles",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": (achieved / peak) if achieved else None, "traffic": None,
                      "avg_launch_ms": avg_launch_s * 1e3, "flops_per_launch": per_launch_flops},
@@ -430,7 +433,8 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("workload_walkers") == args.walkers and tr.get("path") == main_path:
+        if (tr.get("workload_walkers") == args.walkers and tr.get("path") == main_path
+                and tr.get("tile") == r["tiles"][0]):
             out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
             out["roofline"]["algorithmic_bytes"] = tr.get("algorithmic_bytes_per_launch")
     if copy_cal:
@@ -442,6 +446,7 @@ def main():
             "energy_per_site": (rr["energy"] / Ls).real,
             "energy_per_site_err": rr["err"] / Ls,
             "energy_blocks": rr["nblocks"],
+            "gemm_tiles": {"propagate": rr["tiles"][0], "overlap": rr["tiles"][1], "green": rr["tiles"][2]},
             "tflops_overall": fl_ws * rr["walker_steps"] / rr["elapsed"] / 1e12,
             "kernel_ms": {k: round(v[0], 3) for k, v in rr["timing"].items() if v[1]},
         }

@@ -120,10 +120,10 @@ int afqmc_cp_get_ET(afqmc_cp cp, double* ET);
 
 /* GEMM tile shapes of the step (no reference counterpart): tiles[0] the one-body propagator,
  * tiles[1] the overlap, tiles[2] the Green's function GEMM; 0 = 64x64, 1 = 32x64, 2 = 64x32,
- * 3 = 32x32 output tiles (the Green GEMM takes 0 or 1).  autotune times every shape on this
- * walker set's own GEMMs (reps launches each, walker state unchanged), keeps the fastest and
- * returns the choices in tiles (3) and the times in ms (3 x 4, ms per launch, -1 = not a
- * candidate); either output may be NULL. */
+ * 3 = 32x32, 4 = 128x64 output tiles, 5 = 64x64 with a 32-deep K tile (the Green GEMM takes the
+ * 64-column shapes 0, 1, 4, 5).  autotune times every shape on this walker set's own GEMMs (reps
+ * launches each, walker state unchanged), keeps the fastest and returns the choices in tiles (3)
+ * and the times in ms (3 x 6, ms per launch, -1 = not a candidate); either output may be NULL. */
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles);
 int afqmc_cp_autotune(afqmc_cp cp, afqmc_walkers w, int reps, int* tiles, double* ms);
 

@@ -52,9 +52,9 @@ def block_of_work(prop, ws):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_every_tile_shape_gives_the_same_step(precision):
     ref = None
-    for t in range(4):
+    for t in range(6):
         prop, ws = make(ctx(), 4, 4, 5, 4.0, precision=precision)
-        prop.set_tiles(propagate=t, overlap=t, green=t if t < 2 else 0)
+        prop.set_tiles(propagate=t, overlap=t, green=t if t in (0, 1, 4, 5) else 0)
         for _ in range(2):
             prop.step(ws)
         prop.measure(ws)
@@ -75,7 +75,7 @@ def test_autotune_keeps_state_and_results():
     after = ws_a.download()
     for x, y in zip(before, after):
         assert np.array_equal(x, y)
-    assert tiles.shape == (3,) and all(0 <= t < 4 for t in tiles) and tiles[2] in (0, 1)
+    assert tiles.shape == (3,) and all(0 <= t < 6 for t in tiles) and tiles[2] in (0, 1, 4, 5)
     assert np.all(ms[:, 0] > 0) and ms[2, 2] == -1.0 and ms[2, 3] == -1.0
     for _ in range(3):
         prop_a.step(ws_a)
