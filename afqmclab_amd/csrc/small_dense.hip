@@ -1,20 +1,27 @@
-// Per-walker N x N dense kernels, register resident, one workgroup per matrix.
+// Per-walker N x N dense kernels: one workgroup of 16 waves per matrix, the matrix register
+// resident (a 112 x 112 complex FP64 matrix is 196 KB, more than a CU's LDS).
 //
-// The N x N complex FP64 matrix (N <= 160; 112 x 112 is 196 KB, more than a
-// CU's LDS) is distributed 2D-cyclically over a TG x TG thread grid: thread
-// (tr, tc) owns rows tr + TG*a and columns tc + TG*b.  Per elimination step one
-// column travels through LDS (double buffered) and the pivot row by in-wave
-// shuffle; the pivot of the next column is found one step ahead by the lanes
-// that own it, so each step has one barrier.
-//
-// gj_inverse_logdet: Gauss-Jordan exchange sweep with partial pivoting (the
-//   pivot of column k is the largest |re|+|im| among rows not yet used, first
-//   index on ties, as izamax in zgetrf).  The pivots equal LU's u_kk, so
-//   logdet = sum log|u_kk| + log(prod u_kk/|u_kk| * sign(perm)) reproduces
-//   logDeterminant (tensor_2d_common_fun.cpp:75-105).  Writes O^{-1}.
-// chol_inverse: S = R^H R (upper R, positive real diagonal), then R^{-1} by
-//   the same sweep without pivoting; returns sum log R_kk (CholeskyQR step of
-//   the walker restabilisation).
+// gj_inverse_logdet: Gauss-Jordan exchange sweep with partial pivoting.  The pivot of column k
+//   is the largest |re|+|im| among rows not yet used, first index on ties, as izamax in zgetrf,
+//   so the pivots equal LU's u_kk and
+//   logdet = sum log|u_kk| + log(prod u_kk/|u_kk| * sign(perm)) reproduces logDeterminant
+//   (tensor_2d_common_fun.cpp:75-105).  Writes O^{-1}.
+//   Column layout: wave w owns columns w + 16c, lane l rows l + 64r, so a column and its pivot
+//   search live in one wave (a DPP reduction, no LDS round trip).  Per step the owner of column
+//   k+1 updates that column first, finds its pivot and publishes the multipliers
+//   m_i = x_i,k+1 / pivot (0 at the pivot row) through LDS; every wave reads the pivot row of its
+//   own columns with readlane.  A row chosen as pivot is not rescaled: it keeps the scale
+//   -pivot (the exchange step is linear in each row), which is divided out when the inverse is
+//   written.  One barrier per step.
+// chol_inverse: S = R^H R (upper R, positive real diagonal) and R^{-1} in one sweep, row
+//   formulation in one array: after the Jacobi scaling S' = D^-1/2 S D^-1/2, step k leaves
+//   row k of the working array unscaled (scale r_kk: R's row in the upper part, the row of
+//   A = R'^{-H} in the lower part) and updates the rows below it; the lower part holds the
+//   Hermitian Schur complement until its column's step, where adding (S_kk + 1) times the
+//   multiplier turns it into A's column (|S_kk| <= 1 after the scaling, so the cancellation
+//   costs no accuracy).  Row layout: wave w owns rows w + 16c, lane l columns l + 64r, so a row
+//   is published by one wave with coalesced LDS writes and R^{-1} = D^-1/2 A^H is written
+//   coalesced.  Returns sum log R_kk (CholeskyQR step of the walker restabilisation).
 #include "small_dense.hpp"
 
 #include <type_traits>
@@ -23,6 +30,7 @@
 namespace {
 
 constexpr int NMAX = 160;
+constexpr int WAVES = 16;
 
 // f(integral_constant<int, 0>), ..., f(integral_constant<int, E-1>)
 template <int E, typename F, int... I>
@@ -50,348 +58,410 @@ __device__ __forceinline__ zd cfms(zd x, zd a, zd b) {
   return cmk<double>(fma(-a.re, b.re, fma(a.im, b.im, x.re)), fma(-a.re, b.im, fma(-a.im, b.re, x.im)));
 }
 
-// Gauss-Jordan with look-ahead: the threads that own column k+1 (TG consecutive lanes of one
-// wave) update that column first during step k, find its pivot with a reduction inside their
-// lane group and publish the column and the pivot row through LDS, then do the rest of their
-// update.  The pivot search is thereby off the critical path of the other waves, and each
-// step has one barrier.  The pivot row index is uniform over the workgroup (read as a scalar),
-// so the pivot row is taken from registers by a scalar branch and an in-wave shuffle.
-template <typename T, int TG, int E>
-__global__ __launch_bounds__(TG * TG) void gj_la_kernel(SmallDenseArgs p) {
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ zd readlane_z(zd v, int lane) { return cmk<double>(readlane_d(v.re, lane), readlane_d(v.im, lane)); }
+
+// one DPP step of a wave maximum: v = max(v, v of the source lane); lanes without a source (or
+// in rows outside ROW_MASK) keep their value
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_max(unsigned long long v) {
+  const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+  const unsigned nlo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  const unsigned nhi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  const unsigned long long w = ((unsigned long long)nhi << 32) | nlo;
+  return w > v ? w : v;
+}
+
+// maximum over the 64 lanes, returned uniform: row_shr 1, 2, 4, 8 leave each row's maximum in
+// its lane 15; row_bcast 15 and 31 carry it into lane 63
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  v = dpp_max<0x111, 0xf>(v);
+  v = dpp_max<0x112, 0xf>(v);
+  v = dpp_max<0x114, 0xf>(v);
+  v = dpp_max<0x118, 0xf>(v);
+  v = dpp_max<0x142, 0xa>(v);
+  v = dpp_max<0x143, 0xc>(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Gauss-Jordan, column layout: x[r][c] = element (lane + 64r, wave + 16c); NR * 64 >= N and
+// NC * 16 >= N.
+template <typename T, int NR, int NC>
+__global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   const int bidx = blockIdx.x;
   if (p.alive && !p.alive[bidx / p.alive_div]) return;
   const int N = p.N;
-  const int tid = threadIdx.x;
-  const int tr = tid % TG, tc = tid / TG;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   typedef cplx<T> ct;
   const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
   ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
 
-  __shared__ zd colbuf[2][NMAX];
+  __shared__ zd colbuf[2][64 * NR];
   __shared__ int pivrow[2];
-  __shared__ zd pivinv[2];
   __shared__ zd pivs[NMAX];
+  __shared__ zd rowscale[NMAX];
   __shared__ int perm[NMAX];
   __shared__ int qinv[NMAX];
-  __shared__ unsigned char seen[NMAX];
+  __shared__ int singular, parity;
 
-  zd x[E][E];
+  zd x[NR][NC];
 #pragma unroll
-  for (int a = 0; a < E; ++a)
+  for (int r = 0; r < NR; ++r)
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      int i = tr + TG * a, j = tc + TG * c;
-      x[a][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+    for (int c = 0; c < NC; ++c) {
+      const int i = lane + 64 * r, j = wv + WAVES * c;
+      x[r][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
     }
-  unsigned used = 0u;  // bit a: row tr + TG*a has been a pivot row
-  int singular = 0;
+  unsigned used = 0u;  // bit r: row lane + 64r has been a pivot row
+  if (threadIdx.x == 0) { singular = 0; parity = 0; }
 
-  // The owners of column `col` (TG consecutive lanes) publish it: value(a) gives their entry in
-  // register row a; the pivot row is found by a reduction inside the lane group, and the lane
-  // holding the pivot publishes its index and inverse (0 for a zero pivot).
-  auto publish_col = [&](auto value, int col, int buf) {
+  // The owner of column `col` (register column C1 of this wave, already updated through step
+  // col-1) picks its pivot among the unused rows and publishes the multipliers and the pivot.
+  auto publish = [&](auto c1c, int col, int buf) {
+    constexpr int C1 = decltype(c1c)::value;
     unsigned long long key = 0ull;
 #pragma unroll
-    for (int a = 0; a < E; ++a) {
-      const int i = tr + TG * a;
-      if (i < N) {
-        const zd v = value(a);
-        colbuf[buf][i] = v;
-        if (!((used >> a) & 1u)) {
-          const unsigned long long kk = pivot_key(v, i);
-          key = kk > key ? kk : key;
-        }
+    for (int r = 0; r < NR; ++r) {
+      const int i = lane + 64 * r;
+      if (i < N && !((used >> r) & 1u)) {
+        const unsigned long long kk = pivot_key(x[r][C1], i);
+        key = kk > key ? kk : key;
       }
     }
-#pragma unroll
-    for (int o = TG / 2; o > 0; o >>= 1) {
-      const unsigned long long ok = (unsigned long long)__shfl_xor((long long)key, o, 64);
-      key = ok > key ? ok : key;
-    }
+    key = wave_max_u64(key);
     // bit 63 marks a candidate: at step col there are N - col >= 1 unused rows
     const int bi = 255 - (int)(key & 0xFFull);
-    const int prn = (bi >= 0 && bi < N) ? bi : 0;
-    if (tr == prn % TG) {
-      const zd pv = colbuf[buf][prn];  // written by this lane above
-      pivrow[buf] = prn;
-      pivinv[buf] = abs1(pv) > 0.0 ? cinv(pv) : cmk<double>(0, 0);
+    const int pr = (bi >= 0 && bi < N) ? bi : 0;
+    const int rr = pr >> 6, pl = pr & 63;
+    zd piv = cmk<double>(0, 0);
+    static_for<NR>([&](auto rc) {
+      constexpr int R = decltype(rc)::value;
+      if (rr == R) piv = readlane_z(x[R][C1], pl);
+    });
+    const bool nz = abs1(piv) > 0.0;
+    const double id = nz ? 1.0 / (piv.re * piv.re + piv.im * piv.im) : 0.0;
+    const zd inv = cmk<double>(piv.re * id, -piv.im * id);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int i = lane + 64 * r;
+      colbuf[buf][i] = (i == pr) ? cmk<double>(0, 0) : cmul(x[r][C1], inv);
+    }
+    if (lane == 0) {
+      pivrow[buf] = pr;
+      pivs[col] = piv;
+      perm[col] = pr;
+      rowscale[pr] = cmk<double>(-inv.re, -inv.im);
+      if (!nz) singular = 1;
     }
   };
 
-  if (tc == 0) publish_col([&](int a) { return x[a][0]; }, 0, 0);
+  if (wv == 0) publish(std::integral_constant<int, 0>{}, 0, 0);
   __syncthreads();
 
-  // Step k = CB*TG + t: column k sits in register column CB (a compile-time constant), column
-  // k+1 in CB or CB+1, so no register array is ever indexed by a run-time value.
+  // Step k = CB*16 + t: column k is register column CB of wave t, column k+1 register column CB
+  // of wave t+1 (or CB+1 of wave 0), so no register array is indexed by a run-time value.
   auto step = [&](auto cbc, int t) {
     constexpr int CB = decltype(cbc)::value;
-    const int k = CB * TG + t;
-    const int buf = k & 1;
+    const int k = CB * WAVES + t;
+    const int buf = k & 1, nbuf = buf ^ 1;
     const int pr = __builtin_amdgcn_readfirstlane(pivrow[buf]);
-    const zd piv = colbuf[buf][pr];
-    const zd inv = pivinv[buf];
-    if (!(abs1(piv) > 0.0)) singular = 1;
-    if (tid == 0) {
-      perm[k] = pr;
-      pivs[k] = piv;
-    }
-    const int ar = pr / TG, prm = pr % TG;
-    const bool prow = (tr == prm);
-    if (prow) used |= 1u << ar;
-    // rowv[c] = x[pr][j] / piv (ar is uniform, so these are scalar branches around the shuffles);
-    // the update below leaves column k to its own fix-up
-    zd rowv[E];
-    {
-      const int src = prm + TG * (tc % (64 / TG));
+    const int rr = pr >> 6, pl = pr & 63;
+    zd m[NR];
 #pragma unroll
-      for (int a = 0; a < E; ++a) {
-        if (a == ar) {
+    for (int r = 0; r < NR; ++r) m[r] = colbuf[buf][lane + 64 * r];
+    if (lane == pl) used |= 1u << rr;
+    // the pivot row of this wave's columns (uniform)
+    zd rv[NC];
+    static_for<NR>([&](auto rc) {
+      constexpr int R = decltype(rc)::value;
+      if (rr == R) {
 #pragma unroll
-          for (int c = 0; c < E; ++c) {
-            zd v;
-            v.re = __shfl(x[a][c].re, src, 64);
-            v.im = __shfl(x[a][c].im, src, 64);
-            rowv[c] = cmul(v, inv);
-          }
+        for (int c = 0; c < NC; ++c) rv[c] = readlane_z(x[R][c], pl);
+      }
+    });
+    const bool next_here = (k + 1 < N) && (wv == (t + 1) % WAVES);
+    const bool next_in_cb = (t + 1 < WAVES);
+    // look-ahead: column k+1 first, then its pivot and multipliers
+    if (next_here) {
+      if (next_in_cb) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) x[r][CB] = cfms(x[r][CB], m[r], rv[CB]);
+        publish(std::integral_constant<int, CB>{}, k + 1, nbuf);
+      } else {
+        if constexpr (CB + 1 < NC) {
+#pragma unroll
+          for (int r = 0; r < NR; ++r) x[r][CB + 1] = cfms(x[r][CB + 1], m[r], rv[CB + 1]);
+          publish(std::integral_constant<int, CB + 1>{}, k + 1, nbuf);
         }
       }
     }
-    zd cv[E];
+    const bool owner = (wv == t);
 #pragma unroll
-    for (int a = 0; a < E; ++a) {
-      const int i = tr + TG * a;
-      cv[a] = i < N ? colbuf[buf][i] : cmk<double>(0, 0);
+    for (int c = 0; c < NC; ++c) {
+      const bool done = next_here && ((next_in_cb && c == CB) || (!next_in_cb && c == CB + 1));
+      if (done || (owner && c == CB)) continue;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) x[r][c] = cfms(x[r][c], m[r], rv[c]);
     }
-    // look-ahead: the owners of column k+1 compute its updated values first and publish them
-    // (the update below recomputes the same values in place)
-    auto lookahead = [&](auto c1c) {
-      constexpr int C1 = decltype(c1c)::value;
-      publish_col([&](int a) {
-        return (prow && a == ar) ? cmk<double>(-rowv[C1].re, -rowv[C1].im) : cfms(x[a][C1], cv[a], rowv[C1]);
-      }, k + 1, buf ^ 1);
-    };
-    if (k + 1 < N && tc == (t + 1) % TG) {
-      if (t + 1 < TG) {
-        lookahead(std::integral_constant<int, CB>{});
-      } else {
-        if constexpr (CB + 1 < E) lookahead(std::integral_constant<int, CB + 1>{});
-      }
-    }
-    // rank-1 update, then the pivot row and column k
+    // column k becomes the multipliers, -1 at the (unscaled) pivot row
+    if (owner) {
 #pragma unroll
-    for (int c = 0; c < E; ++c)
-#pragma unroll
-      for (int a = 0; a < E; ++a) x[a][c] = cfms(x[a][c], cv[a], rowv[c]);
-    if (prow) {
-#pragma unroll
-      for (int a = 0; a < E; ++a)
-        if (a == ar)
-#pragma unroll
-          for (int c = 0; c < E; ++c) x[a][c] = cmk<double>(-rowv[c].re, -rowv[c].im);
-    }
-    if (tc == t) {
-#pragma unroll
-      for (int a = 0; a < E; ++a) x[a][CB] = (tr + TG * a == pr) ? inv : cmul(cv[a], inv);
+      for (int r = 0; r < NR; ++r) x[r][CB] = (lane + 64 * r == pr) ? cmk<double>(-1.0, 0.0) : m[r];
     }
     __syncthreads();
   };
   auto block = [&](auto cbc) {
     constexpr int CB = decltype(cbc)::value;
-    for (int t = 0; t < TG && CB * TG + t < N; ++t) step(cbc, t);
+    for (int t = 0; t < WAVES && CB * WAVES + t < N; ++t) step(cbc, t);
   };
-  static_for<E>(block);
-  if (tid < N) qinv[perm[tid]] = tid;
-  if (tid == 0) {
-    double lognorm = 0.0;
-    zd phase = cmk<double>(1.0, 0.0);
-    if (!singular) {
-      for (int k = 0; k < N; ++k) {
+  static_for<NC>(block);
+
+  if ((int)threadIdx.x < N) qinv[perm[threadIdx.x]] = threadIdx.x;
+  // sign of the row permutation: parity of its inversions, counted in parallel
+  {
+    int odd = 0;
+    for (int e = threadIdx.x; e < N * N; e += 64 * WAVES) {
+      const int i = e / N, j = e - i * N;
+      odd ^= (i < j && perm[i] > perm[j]) ? 1 : 0;
+    }
+    if (odd) atomicXor(&parity, 1);
+  }
+  // log|det| and the phase of det from the pivots: wave 0, fixed-order trees (deterministic)
+  double lognorm = 0.0;
+  zd phase = cmk<double>(1.0, 0.0);
+  if (wv == 0) {
+#pragma unroll
+    for (int q = 0; q < (NMAX + 63) / 64; ++q) {
+      const int k = lane + 64 * q;
+      if (k < N) {
         const zd pv = pivs[k];
         const double ap = cabs(pv);
+        const double ia = ap > 0.0 ? 1.0 / ap : 0.0;
         lognorm += log(ap);
-        phase = cmul(phase, cmk<double>(pv.re / ap, pv.im / ap));
+        phase = cmul(phase, cmk<double>(pv.re * ia, pv.im * ia));
       }
     }
-    int visited_parity = 0;
-    for (int i = 0; i < N; ++i) seen[i] = 0;
-    for (int i = 0; i < N; ++i) {
-      if (seen[i]) continue;
-      int len = 0, j = i;
-      while (len < N && !seen[j]) { seen[j] = 1; j = perm[j]; ++len; }
-      visited_parity ^= (len - 1) & 1;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lognorm += __shfl_xor(lognorm, o, 64);
+      const zd q = cmk<double>(__shfl_xor(phase.re, o, 64), __shfl_xor(phase.im, o, 64));
+      phase = cmul(phase, q);
     }
-    zd ph = visited_parity ? cmk<double>(-phase.re, -phase.im) : phase;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const zd ph = parity ? cmk<double>(-phase.re, -phase.im) : phase;
     zd ld;
     if (singular) {
       ld = cmk<double>(-1e300, 0.0);
     } else {
-      ld = cmk<double>(lognorm + 0.0, atan2(ph.im, ph.re));
+      ld = cmk<double>(lognorm, atan2(ph.im, ph.re));
       ld.re += log(hypot(ph.re, ph.im));
     }
     p.logdet[bidx] = ld;
     if (p.info) p.info[bidx] = singular;
   }
   if (!p.write_inverse) return;
-  __syncthreads();
+  // inverse: O^{-1}[qinv[i]][perm[j]] = x[i][j] / (-pivot of row i)
 #pragma unroll
-  for (int a = 0; a < E; ++a) {
-    int i = tr + TG * a;
+  for (int r = 0; r < NR; ++r) {
+    const int i = lane + 64 * r;
     if (i >= N) continue;
-    int row = qinv[i];
+    const int row = qinv[i];
+    const zd sc = rowscale[i];
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      int j = tc + TG * c;
+    for (int c = 0; c < NC; ++c) {
+      const int j = wv + WAVES * c;
       if (j >= N) continue;
-      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(x[a][c]);
+      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(x[r][c], sc));
     }
   }
 }
 
-template <typename T, int TG, int E>
-__global__ __launch_bounds__(TG * TG) void chol_kernel(SmallDenseArgs p) {
+// Cholesky and R^{-1}, row layout: y[c][r] = element (wave + 16c, lane + 64r).
+template <typename T, int NR, int NC>
+__global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
   const int bidx = blockIdx.x;
   if (p.alive && !p.alive[bidx / p.alive_div]) return;
   const int N = p.N;
-  const int tid = threadIdx.x;
-  const int tr = tid % TG, tc = tid / TG;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   typedef cplx<T> ct;
   const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
   ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
 
-  __shared__ zd colbuf[2][NMAX];
-  __shared__ zd rowbuf[2][NMAX];
+  __shared__ zd rowbuf[2][64 * NR];
+  __shared__ double iskkbuf[2];
+  __shared__ double dsq[64 * NR];  // D^-1/2
+  __shared__ double skk_all[NMAX];  // r'_kk^2
+  __shared__ double logr[NMAX];     // log R_kk
+  __shared__ int bad;
 
-  zd x[E][E];
+  // S is Hermitian: element (i, j) is read as conj(S(j, i)), so consecutive lanes read
+  // consecutive addresses
+  zd y[NC][NR];
 #pragma unroll
-  for (int a = 0; a < E; ++a)
+  for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      int i = tr + TG * a, j = tc + TG * c;
-      x[a][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+    for (int r = 0; r < NR; ++r) {
+      const int i = wv + WAVES * c, j = lane + 64 * r;
+      y[c][r] = (i < N && j < N) ? cconj(to_zd(in[j + (long long)i * p.ld_in])) : cmk<double>(0, 0);
     }
-  double logsum = 0.0, logsub = 0.0;
-  int bad = 0;
-  // Cholesky, upper factor: one barrier per step (double-buffered row)
-  for (int k = 0; k < N; ++k) {
-    const int buf = k & 1;
-    if (tr == k % TG) {
-      const int ar = k / TG;
-#pragma unroll
-      for (int a = 0; a < E; ++a) {
-        if (a != ar) continue;
-#pragma unroll
-        for (int c = 0; c < E; ++c) {
-          int j = tc + TG * c;
-          if (j < N) rowbuf[buf][j] = x[a][c];
-        }
-      }
-    }
-    __syncthreads();
-    double skk = rowbuf[buf][k].re;
-    if (!(skk > 0.0)) { bad = 1; skk = 1.0; }
-    const double d = sqrt(skk), id = 1.0 / d;
-    if (tid == 0) {
-      double l = log(d);
-      logsum += l;
-      if (k < p.nsub) logsub += l;
-    }
-#pragma unroll
-    for (int a = 0; a < E; ++a) {
-      int i = tr + TG * a;
-      if (i >= N || i < k) continue;
-      zd rki = cscale(rowbuf[buf][i], id);  // R[k,i]
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        int j = tc + TG * c;
-        if (j >= N || j < k) continue;
-        zd rkj = cscale(rowbuf[buf][j], id);
-        if (i == k) x[a][c] = rkj;
-        else x[a][c] = csub(x[a][c], cmul(cconj(rki), rkj));
-      }
-    }
-  }
-  // zero the strictly lower part
-#pragma unroll
-  for (int a = 0; a < E; ++a)
-#pragma unroll
-    for (int c = 0; c < E; ++c) {
-      int i = tr + TG * a, j = tc + TG * c;
-      if (j < i) x[a][c] = cmk<double>(0, 0);
-    }
-  // R^{-1} by the exchange sweep with pivot (k, k); the pivot row reaches each thread by an
-  // in-wave shuffle (as in gj_la_kernel), so one barrier per step
+  if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  for (int k = 0; k < N; ++k) {
-    const int buf = k & 1;
-    if (tc == k % TG) {
-      const int kc = k / TG;
 #pragma unroll
-      for (int a = 0; a < E; ++a) {
-        int i = tr + TG * a;
+  for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int c = 0; c < E; ++c)
-          if (c == kc && i < N) colbuf[buf][i] = x[a][c];
+    for (int r = 0; r < NR; ++r) {
+      const int i = wv + WAVES * c, j = lane + 64 * r;
+      if (i == j && i < N) {
+        double d = y[c][r].re;
+        if (!(d > 0.0)) { bad = 1; d = 1.0; }
+        dsq[i] = 1.0 / sqrt(d);
+        logr[i] = 0.5 * log(d);
       }
     }
-    __syncthreads();
-    const zd inv = cinv(colbuf[buf][k]);
-    zd rowv[E];
-    {
-      const int ar = k / TG;
-      const int src = (k % TG) + TG * (tc % (64 / TG));
+  __syncthreads();
+  double dcol[NR];
 #pragma unroll
-      for (int c = 0; c < E; ++c) {
-        zd v = cmk<double>(0, 0);
+  for (int r = 0; r < NR; ++r) {
+    const int j = lane + 64 * r;
+    dcol[r] = j < N ? dsq[j] : 0.0;
+  }
 #pragma unroll
-        for (int a = 0; a < E; ++a)
-          if (a == ar) v = x[a][c];
-        v.re = __shfl(v.re, src, 64);
-        v.im = __shfl(v.im, src, 64);
-        rowv[c] = cmul(v, inv);
-      }
+  for (int c = 0; c < NC; ++c) {
+    const int i = wv + WAVES * c;
+    if (i >= N) continue;
+    const double di = dsq[i];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) y[c][r] = cscale(y[c][r], di * dcol[r]);
+  }
+
+  // The owner of row k (register row CK of this wave, updated through step k-1) publishes it.
+  auto publish = [&](auto ckc, int k, int buf) {
+    constexpr int CK = decltype(ckc)::value;
+    const int kr = k >> 6, kl = k & 63;
+    double skk = 1.0;
+    static_for<NR>([&](auto rc) {
+      constexpr int R = decltype(rc)::value;
+      if (kr == R) skk = readlane_d(y[CK][R].re, kl);
+    });
+    const bool ok = skk > 0.0;
+    if (!ok) skk = 1.0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int j = lane + 64 * r;
+      rowbuf[buf][j] = (j == k) ? cmk<double>(skk + 1.0, 0.0) : y[CK][r];
     }
+    if (lane == 0) {
+      iskkbuf[buf] = 1.0 / skk;
+      skk_all[k] = skk;
+      if (!ok) bad = 1;
+    }
+  };
+
+  if (wv == 0) publish(std::integral_constant<int, 0>{}, 0, 0);
+  __syncthreads();
+
+  // Step k = CB*16 + t: row k is register row CB of wave t, row k+1 register row CB of wave t+1
+  // (or CB+1 of wave 0).
+  auto step = [&](auto cbc, int t) {
+    constexpr int CB = decltype(cbc)::value;
+    const int k = CB * WAVES + t;
+    const int buf = k & 1, nbuf = buf ^ 1;
+    const double iskk = iskkbuf[buf];
+    zd rk[NR];
 #pragma unroll
-    for (int a = 0; a < E; ++a) {
-      int i = tr + TG * a;
-      if (i >= N) continue;
-      zd ci = colbuf[buf][i];
+    for (int r = 0; r < NR; ++r) rk[r] = rowbuf[buf][lane + 64 * r];
+    const bool next_here = (k + 1 < N) && (wv == (t + 1) % WAVES);
+    const bool next_in_cb = (t + 1 < WAVES);
+    auto update_row = [&](auto cc) {
+      constexpr int C = decltype(cc)::value;
+      const int i = wv + WAVES * C;
+      const zd mi = cscale(cconj(rowbuf[buf][i]), iskk);
 #pragma unroll
-      for (int c = 0; c < E; ++c) {
-        int j = tc + TG * c;
-        if (j >= N) continue;
-        zd r = rowv[c];
-        if (i == k) {
-          x[a][c] = (j == k) ? inv : cmk<double>(-r.re, -r.im);
-        } else if (j == k) {
-          x[a][c] = cmul(ci, inv);
-        } else {
-          x[a][c] = csub(x[a][c], cmul(ci, r));
+      for (int r = 0; r < NR; ++r) y[C][r] = cfms(y[C][r], mi, rk[r]);
+    };
+    if (next_here) {
+      if (next_in_cb) {
+        update_row(std::integral_constant<int, CB>{});
+        publish(std::integral_constant<int, CB>{}, k + 1, nbuf);
+      } else {
+        if constexpr (CB + 1 < NC) {
+          update_row(std::integral_constant<int, CB + 1>{});
+          publish(std::integral_constant<int, CB + 1>{}, k + 1, nbuf);
         }
       }
     }
-  }
-  if (tid == 0) {
-    p.logdet[bidx] = cmk<double>(logsum, logsub);
-    if (p.info) p.info[bidx] = bad;
-  }
+    static_for<NC>([&](auto cc) {
+      constexpr int C = decltype(cc)::value;
+      const int i = wv + WAVES * C;
+      const bool done = next_here && ((next_in_cb && C == CB) || (!next_in_cb && C == CB + 1));
+      if (i > k && i < N && !done) update_row(cc);
+    });
+    __syncthreads();
+  };
+  auto block = [&](auto cbc) {
+    constexpr int CB = decltype(cbc)::value;
+    for (int t = 0; t < WAVES && CB * WAVES + t < N; ++t) step(cbc, t);
+  };
+  static_for<NC>(block);
+
+  // sum log R_kk (and over k < nsub): wave 0, fixed-order tree (deterministic)
+  if (wv == 0) {
+    double ls = 0.0, lsub = 0.0;
 #pragma unroll
-  for (int a = 0; a < E; ++a) {
-    int i = tr + TG * a;
+    for (int q = 0; q < (NMAX + 63) / 64; ++q) {
+      const int k = lane + 64 * q;
+      if (k < N) {
+        const double l = logr[k] + 0.5 * log(skk_all[k]);
+        ls += l;
+        if (k < p.nsub) lsub += l;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ls += __shfl_xor(ls, o, 64);
+      lsub += __shfl_xor(lsub, o, 64);
+    }
+    if (lane == 0) {
+      p.logdet[bidx] = cmk<double>(ls, lsub);
+      if (p.info) p.info[bidx] = bad;
+    }
+  }
+  // R^{-1}(j, i) = dsq[j] conj(A(i, j)) for j < i with A(i, j) = y(i, j) / r'_ii; dsq[i] / r'_ii
+  // on the diagonal; zero below
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int i = wv + WAVES * c;
     if (i >= N) continue;
+    const double ri = 1.0 / sqrt(skk_all[i]);
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      int j = tc + TG * c;
+    for (int r = 0; r < NR; ++r) {
+      const int j = lane + 64 * r;
       if (j >= N) continue;
-      out[i + (long long)j * p.ld_out] = from_zd<T>(x[a][c]);
+      zd v;
+      if (j < i) v = cscale(cconj(y[c][r]), ri * dcol[r]);
+      else if (j == i) v = cmk<double>(ri * dcol[r], 0.0);
+      else v = cmk<double>(0, 0);
+      out[j + (long long)i * p.ld_out] = from_zd<T>(v);
     }
   }
 }
 
-template <typename T, int TG, int E>
+template <typename T, int NR, int NC>
 int run(int kind, const SmallDenseArgs& a, hipStream_t s) {
-  dim3 grid(a.batch), block(TG * TG);
-  if (kind == 0) hipLaunchKernelGGL((gj_la_kernel<T, TG, E>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((chol_kernel<T, TG, E>), grid, block, 0, s, a);
+  dim3 grid(a.batch), block(64 * WAVES);
+  if (kind == 0) hipLaunchKernelGGL((gj_kernel<T, NR, NC>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((chol_kernel<T, NR, NC>), grid, block, 0, s, a);
   HIP_LAUNCH_CHECK();
   return 0;
 }
@@ -401,11 +471,13 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
   REQUIRE(a.N >= 1 && a.N <= NMAX, "small dense kernels support 1 <= N <= %d (got %d)", NMAX, a.N);
   REQUIRE(a.alive_div >= 1, "alive_div must be >= 1");
   if (a.batch <= 0) return 0;
-  if (a.N <= 16) return run<T, 8, 2>(kind, a, s);
-  if (a.N <= 32) return run<T, 16, 2>(kind, a, s);
-  if (a.N <= 64) return run<T, 16, 4>(kind, a, s);
-  if (a.N <= 128) return run<T, 32, 4>(kind, a, s);
-  return run<T, 32, 5>(kind, a, s);
+  if (a.N <= 16) return run<T, 1, 1>(kind, a, s);
+  if (a.N <= 32) return run<T, 1, 2>(kind, a, s);
+  if (a.N <= 64) return run<T, 1, 4>(kind, a, s);
+  if (a.N <= 96) return run<T, 2, 6>(kind, a, s);
+  if (a.N <= 112) return run<T, 2, 7>(kind, a, s);
+  if (a.N <= 128) return run<T, 2, 8>(kind, a, s);
+  return run<T, 3, 10>(kind, a, s);
 }
 
 }  // namespace
