@@ -1,12 +1,19 @@
 // Batched complex GEMM kernels (MFMA 16x16x4, f64 or f32) for gfx950.
 //
-// The complex product is done with four real MFMAs per 16x16x4 block:
-//   Cre += Are*Bre - Aim*Bim,  Cim += Are*Bim + Aim*Bre
-// (8 real flops per complex multiply-add, as counted in the roofline).  Tiles
-// are staged through LDS as separate re/im planes so that each MFMA operand
-// is one LDS read per lane; the next K tile is prefetched into registers while
-// the current one is multiplied.
+// FP32: the complex product takes three real MFMAs per 16x16x4 block (the 3M
+// form of Gauss's product): P1 += Are*Bre, P2 += Aim*Bim,
+// P3 += (Are+Aim)*(Bre+Bim), accumulated over K, then Cre = P1 - P2,
+// Cim = P3 - P1 - P2: 6 real flops per complex multiply-add where the roofline
+// counts the algorithmic 8.  FP64: four (Cre += Are*Bre - Aim*Bim,
+// Cim += Are*Bim + Aim*Bre); a third set of FP64 accumulators would cost more
+// occupancy than the saved MFMA time.  A real A (op NR) needs two products,
+// Cre += A*Bre and Cim += A*Bim.  Tiles are staged
+// through LDS as separate re/im planes so that each MFMA operand is one LDS
+// read per lane (the operand sums are formed in registers); the next K tile is
+// prefetched into registers while the current one is multiplied.
 #include "gemm.hpp"
+
+#include <type_traits>
 
 namespace {
 
@@ -59,13 +66,16 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
   const ct* __restrict__ Bm = reinterpret_cast<const ct*>(p.B) + (long long)b * p.strideB;
   const ct* __restrict__ S = SCALE ? reinterpret_cast<const ct*>(p.scale) + (long long)b * p.scale_strideB : nullptr;
 
-  acc_t acc_re[FM][FN], acc_im[FM][FN];
+  // 3M (FP32, complex A): acc_re = P1, acc_x = P2, acc_im = P3 until the end of the K loop
+  constexpr bool GAUSS = std::is_same<T, float>::value && OPA != OP_NR;
+  acc_t acc_re[FM][FN], acc_im[FM][FN], acc_x[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       acc_re[i][j] = acc_t{0, 0, 0, 0};
       acc_im[i][j] = acc_t{0, 0, 0, 0};
+      acc_x[i][j] = acc_t{0, 0, 0, 0};
     }
 
   ct ra[A_ELEMS], rb[B_ELEMS];
@@ -129,28 +139,36 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
     if (kt + 1 < nk) load_tiles((kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T are[FM], aim[FM], naim[FM], bre[FN], bim[FN];
+      T are[FM], aim[FM], asum[FM], bre[FN], bim[FN], bsum[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         int m = wm * TM + i * 16 + lr;
         are[i] = As[0][kk + lk][m];
         aim[i] = As[1][kk + lk][m];
-        naim[i] = -aim[i];
+        if (GAUSS) asum[i] = are[i] + aim[i];
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         int n = wn * TN + j * 16 + lr;
         bre[j] = Bs[0][kk + lk][n];
         bim[j] = Bs[1][kk + lk][n];
+        if (GAUSS) bsum[j] = bre[j] + bim[j];
       }
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          acc_re[i][j] = mfma_traits<T>::mma(are[i], bre[j], acc_re[i][j]);
-          acc_im[i][j] = mfma_traits<T>::mma(are[i], bim[j], acc_im[i][j]);
-          if (OPA != OP_NR) {  // real A: the two products with Im(A) vanish
-            acc_re[i][j] = mfma_traits<T>::mma(naim[i], bim[j], acc_re[i][j]);
+          if (OPA == OP_NR) {  // real A: two products
+            acc_re[i][j] = mfma_traits<T>::mma(are[i], bre[j], acc_re[i][j]);
+            acc_im[i][j] = mfma_traits<T>::mma(are[i], bim[j], acc_im[i][j]);
+          } else if (GAUSS) {
+            acc_re[i][j] = mfma_traits<T>::mma(are[i], bre[j], acc_re[i][j]);
+            acc_x[i][j] = mfma_traits<T>::mma(aim[i], bim[j], acc_x[i][j]);
+            acc_im[i][j] = mfma_traits<T>::mma(asum[i], bsum[j], acc_im[i][j]);
+          } else {
+            acc_re[i][j] = mfma_traits<T>::mma(are[i], bre[j], acc_re[i][j]);
+            acc_im[i][j] = mfma_traits<T>::mma(are[i], bim[j], acc_im[i][j]);
+            acc_re[i][j] = mfma_traits<T>::mma(-aim[i], bim[j], acc_re[i][j]);
             acc_im[i][j] = mfma_traits<T>::mma(aim[i], bre[j], acc_im[i][j]);
           }
         }
@@ -160,6 +178,19 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
       store_tiles();
       __syncthreads();
     }
+  }
+
+  if (GAUSS) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const T p1 = acc_re[i][j][r], p2 = acc_x[i][j][r], p3 = acc_im[i][j][r];
+          acc_re[i][j][r] = p1 - p2;
+          acc_im[i][j][r] = (p3 - p1) - p2;
+        }
   }
 
   if (EPI == EPI_STORE) {

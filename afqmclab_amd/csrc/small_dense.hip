@@ -58,6 +58,13 @@ __device__ __forceinline__ zd cfms(zd x, zd a, zd b) {
   return cmk<double>(fma(-a.re, b.re, fma(a.im, b.im, x.re)), fma(-a.re, b.im, fma(-a.im, b.re, x.im)));
 }
 
+// 1/d for a positive normal d: hardware reciprocal refined by two Newton steps
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, lane);
@@ -147,7 +154,7 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
       if (rr == R) piv = readlane_z(x[R][C1], pl);
     });
     const bool nz = abs1(piv) > 0.0;
-    const double id = nz ? 1.0 / (piv.re * piv.re + piv.im * piv.im) : 0.0;
+    const double id = nz ? recip(piv.re * piv.re + piv.im * piv.im) : 0.0;
     const zd inv = cmk<double>(piv.re * id, -piv.im * id);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -362,7 +369,7 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
       rowbuf[buf][j] = (j == k) ? cmk<double>(skk + 1.0, 0.0) : y[CK][r];
     }
     if (lane == 0) {
-      iskkbuf[buf] = 1.0 / skk;
+      iskkbuf[buf] = recip(skk);
       skk_all[k] = skk;
       if (!ok) bad = 1;
     }
