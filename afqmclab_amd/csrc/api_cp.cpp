@@ -167,6 +167,7 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
       a.B = wf[b].p; a.ldb = Mb; a.strideB = w->wsize(b);
       a.C = cp->qrS[b].p; a.ldc = Nb; a.strideC = (long long)Nb * Nb;
       a.alive = w->alive.p;
+      a.herm = 1;
       if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
       const int* mask = w->alive.p;
       if (pass == 1) {
@@ -187,6 +188,7 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
       g.B = cp->qrS[b].p; g.ldb = Nb; g.strideB = (long long)Nb * Nb;
       g.C = tmp[b].p; g.ldc = Mb; g.strideC = w->wsize(b);
       g.alive = mask;
+      g.b_upper = 1;  // R^{-1} is upper triangular
       if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
       if (pass == 1 &&
           (rc = launch_select_alive(w->nw, w->wsize(b), sizeof(zd), cp->qr_skip.p, wf[b].p, tmp[b].p, s)))

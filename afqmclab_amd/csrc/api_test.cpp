@@ -25,7 +25,7 @@ std::vector<cplx<T>> narrow(const double* p, size_t n) {
 
 template <typename T>
 int zgemm_t(afqmc_ctx ctx, int opa, int M, int N, int K, int batch, const double* A, long long strideA, const double* B,
-            const double* scale, int scale_group, double* C) {
+            const double* scale, int scale_group, int flags, int tile, double* C) {
   hipStream_t s = ctx->stream;
   const long long lda = (opa == OP_N || opa == OP_NR) ? M : K;
   const size_t nA = (size_t)(strideA ? strideA * batch : (long long)M * K);
@@ -53,6 +53,9 @@ int zgemm_t(afqmc_ctx ctx, int opa, int M, int N, int K, int batch, const double
   a.C = dC.p; a.ldc = M; a.strideC = (long long)M * N;
   a.alpha_re = 1.0;
   a.scale_group = 1;
+  a.herm = flags & 1;
+  a.b_upper = (flags >> 1) & 1;
+  a.tile = tile;
   if (scale) {
     // one K-vector per column group of scale_group columns (per batch: N / scale_group groups)
     const int groups = (N + scale_group - 1) / scale_group;
@@ -104,15 +107,17 @@ int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ain
 extern "C" {
 
 int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K, int batch, const double* A,
-                     long long strideA, const double* B, const double* scale, int scale_group, double* C) {
+                     long long strideA, const double* B, const double* scale, int scale_group, int flags, int tile,
+                     double* C) {
   REQUIRE(ctx && A && B && C, "null argument");
+  REQUIRE(flags >= 0 && flags <= 3, "flags: bit 0 Hermitian result, bit 1 upper-triangular B");
   REQUIRE(M > 0 && N > 0 && K > 0 && batch > 0, "sizes must be positive");
   REQUIRE(opa >= 0 && opa <= 3, "opa must be 0 (N), 1 (T), 2 (C) or 3 (N with real A)");
   REQUIRE(!scale || opa == OP_N, "row scaling is supported with op(A) = N");
   REQUIRE(opa != OP_NR || !strideA || strideA > 0, "invalid strideA");
   ScopedDevice sd(ctx);
-  return precision ? zgemm_t<float>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, C)
-                   : zgemm_t<double>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, C);
+  return precision ? zgemm_t<float>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, flags, tile, C)
+                   : zgemm_t<double>(ctx, opa, M, N, K, batch, A, strideA, B, scale, scale_group, flags, tile, C);
 }
 
 int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,

@@ -55,6 +55,7 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
     }
   }
   if (p.alive && !p.alive[b]) return;
+  if (p.herm && bx > by) return;  // the conjugate transpose of tile (by, bx) fills it
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -130,7 +131,10 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
     }
   };
 
-  const int nk = (p.K + BK - 1) / BK;
+  int nk = (p.K + BK - 1) / BK;
+  // upper-triangular B: rows k > n of column n are zero, so K tiles past this tile's last
+  // column add nothing
+  if (p.b_upper) nk = min(nk, (min(n0 + BN, p.K) + BK - 1) / BK);
   load_tiles(0);
   store_tiles();
   __syncthreads();
@@ -214,6 +218,7 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
               yi += (double)o.im;
             }
             C[off] = cmk<T>((T)yr, (T)yi);
+            if (p.herm && bx < by) C[gn + (long long)gm * p.ldc] = cmk<T>((T)yr, (T)-yi);
           }
         }
       }
@@ -317,6 +322,8 @@ int dispatch_tile(const GemmArgs& a, hipStream_t s) {
     case GEMM_TILE_32x32: return run<T, OPA, 32, 32, 16, 2, 2, EPI, SCALE>(a, s);
     case GEMM_TILE_128x64: return run<T, OPA, 128, 64, 16, 2, 2, EPI, SCALE>(a, s);
     case GEMM_TILE_64x64_K32: return run<T, OPA, 64, 64, 32, 2, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_64x32_K32: return run<T, OPA, 64, 32, 32, 2, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_32x64_K32: return run<T, OPA, 32, 64, 32, 2, 2, EPI, SCALE>(a, s);
     default: return run<T, OPA, 64, 64, 16, 2, 2, EPI, SCALE>(a, s);
   }
 }
@@ -332,6 +339,7 @@ int dispatch_green(const GemmArgs& a, hipStream_t s) {
   if (a.tile == GEMM_TILE_32x64) return run<T, OP_N, 32, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
   if (a.tile == GEMM_TILE_128x64) return run<T, OP_N, 128, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
   if (a.tile == GEMM_TILE_64x64_K32) return run<T, OP_N, 64, 64, 32, 2, 2, EPI_GREEN, false>(a, s);
+  if (a.tile == GEMM_TILE_32x64_K32) return run<T, OP_N, 32, 64, 32, 2, 2, EPI_GREEN, false>(a, s);
   return run<T, OP_N, 64, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
 }
 
@@ -346,6 +354,9 @@ int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s) {
   REQUIRE(epi == EPI_STORE || (a.out0 && a.P0), "cgemm: green epilogue needs P0/out0");
   REQUIRE(a.tile >= 0 && a.tile < GEMM_TILE_COUNT, "cgemm: unknown tile shape %d", a.tile);
   REQUIRE(epi != EPI_GREEN || gemm_tile_bn(a.tile) == 64, "cgemm: the Green epilogue needs a BN = 64 tile");
+  REQUIRE(!a.herm || (opa == OP_C && epi == EPI_STORE && !a.beta_one && a.alpha_im == 0.0 && a.M == a.N &&
+                      gemm_tile_square(a.tile)),
+          "cgemm: a Hermitian result needs op C, a square tile, M == N, real alpha and a plain store");
   if (epi == EPI_STORE) {
     if (opa == OP_N) return sc ? dispatch_tile<T, OP_N, EPI_STORE, true>(a, s) : dispatch_tile<T, OP_N, EPI_STORE, false>(a, s);
     if (opa == OP_T) return dispatch_default<T, OP_T, EPI_STORE, false>(a, s);

@@ -35,14 +35,24 @@ struct GemmArgs {
   int n_fastest;
   // output tile shape (GEMM_TILE_*); the Green epilogue supports the BN = 64 shapes only
   int tile;
+  // herm: the result is Hermitian (a Gram matrix op(A) B with op C and A = B): only the tiles on
+  // or above the diagonal are computed, the others written as their conjugate transposes
+  // (square tiles only).  b_upper: B is upper triangular, so the K loop of an output column tile
+  // stops at its last column.
+  int herm, b_upper;
 };
 
 // Output tile shapes of the MFMA GEMM (4 waves each): the autotuner picks one per call site.
 enum {
   GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_64x32 = 2, GEMM_TILE_32x32 = 3, GEMM_TILE_128x64 = 4,
-  GEMM_TILE_64x64_K32 = 5, GEMM_TILE_COUNT = 6
+  GEMM_TILE_64x64_K32 = 5, GEMM_TILE_64x32_K32 = 6, GEMM_TILE_32x64_K32 = 7, GEMM_TILE_COUNT = 8
 };
-inline int gemm_tile_bn(int tile) { return (tile == GEMM_TILE_64x32 || tile == GEMM_TILE_32x32) ? 32 : 64; }
+inline int gemm_tile_bn(int tile) {
+  return (tile == GEMM_TILE_64x32 || tile == GEMM_TILE_32x32 || tile == GEMM_TILE_64x32_K32) ? 32 : 64;
+}
+inline bool gemm_tile_square(int tile) {
+  return tile == GEMM_TILE_64x64 || tile == GEMM_TILE_32x32 || tile == GEMM_TILE_64x64_K32;
+}
 
 template <typename T>
 int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s);

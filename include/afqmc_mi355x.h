@@ -120,10 +120,11 @@ int afqmc_cp_get_ET(afqmc_cp cp, double* ET);
 
 /* GEMM tile shapes of the step (no reference counterpart): tiles[0] the one-body propagator,
  * tiles[1] the overlap, tiles[2] the Green's function GEMM; 0 = 64x64, 1 = 32x64, 2 = 64x32,
- * 3 = 32x32, 4 = 128x64 output tiles, 5 = 64x64 with a 32-deep K tile (the Green GEMM takes the
- * 64-column shapes 0, 1, 4, 5).  autotune times every shape on this walker set's own GEMMs (reps
- * launches each, walker state unchanged), keeps the fastest and returns the choices in tiles (3)
- * and the times in ms (3 x 6, ms per launch, -1 = not a candidate); either output may be NULL. */
+ * 3 = 32x32, 4 = 128x64 output tiles with a 16-deep K tile, 5 = 64x64, 6 = 64x32, 7 = 32x64 with a
+ * 32-deep K tile (the Green GEMM takes the 64-column shapes 0, 1, 4, 5, 7).  autotune times every
+ * shape on this walker set's own GEMMs (reps launches each, walker state unchanged), keeps the
+ * fastest and returns the choices in tiles (3) and the times in ms (3 x 8, ms per launch, -1 = not
+ * a candidate); either output may be NULL. */
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles);
 int afqmc_cp_autotune(afqmc_cp cp, afqmc_walkers w, int reps, int* tiles, double* ms);
 
@@ -280,9 +281,12 @@ int afqmc_ph_adjust_ET(afqmc_ph ph);
  * No reference counterpart: tensor_2d_bl_cpu.cpp's gmm / LUconstruct+inverse /
  * QR are what they are checked against. */
 /* C_b = op(A_b) diag(scale) B_b; A M x K (op N) or K x M (op T/C), strideA = 0 shares A;
- * opa = 3: op N with A real (A then holds real doubles) */
+ * opa = 3: op N with A real (A then holds real doubles).  flags bit 0: the result is Hermitian
+ * (op C, A = B; only tiles on or above the diagonal computed), bit 1: B upper triangular; tile: a
+ * GEMM tile shape (as afqmc_cp_set_tiles) */
 int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K, int batch, const double* A,
-                     long long strideA, const double* B, const double* scale, int scale_group, double* C);
+                     long long strideA, const double* B, const double* scale, int scale_group, int flags, int tile,
+                     double* C);
 /* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting) */
 int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,
                           double* logdet, int* info);

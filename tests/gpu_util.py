@@ -26,9 +26,11 @@ def crand(rng, *shape):
     return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
 
 
-def zgemm(opa, A, B, precision=0, batch=1, strideA=0, scale=None, scale_group=1):
+def zgemm(opa, A, B, precision=0, batch=1, strideA=0, scale=None, scale_group=1, herm=False, b_upper=False,
+          tile=0):
     """C_b = op(A_b) diag(scale) B_b via the device GEMM.  A, B given per batch as 3-D arrays
-    (batch, rows, cols) or 2-D for shared A."""
+    (batch, rows, cols) or 2-D for shared A.  herm: the result is Hermitian (op C with A = B, only
+    the upper tiles computed); b_upper: B is upper triangular."""
     A = np.asarray(A, dtype=np.complex128)
     B = np.asarray(B, dtype=np.complex128)
     if B.ndim == 2:
@@ -49,7 +51,7 @@ def zgemm(opa, A, B, precision=0, batch=1, strideA=0, scale=None, scale_group=1)
     Cbuf = np.zeros((batch, N, M), dtype=np.complex128)
     sbuf = None if scale is None else np.ascontiguousarray(scale, dtype=np.complex128)
     check(lib().afqmc_test_zgemm(ctx().h, precision, opa, M, N, K, batch, dptr(Abuf), strideA, dptr(Bbuf),
-                                 dptr(sbuf), scale_group, dptr(Cbuf)))
+                                 dptr(sbuf), scale_group, int(herm) | (int(b_upper) << 1), tile, dptr(Cbuf)))
     return np.transpose(Cbuf, (0, 2, 1))
 
 

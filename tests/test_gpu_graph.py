@@ -12,6 +12,10 @@ from afqmclab_amd.model import HubbardModel, free_electron_trial, spin_block_tri
 from tests.gpu_util import ctx  # noqa: E402
 
 
+# the Green GEMM's epilogue needs the 64-column tiles
+GREEN_TILES = tuple(t for t, name in enumerate(CPPropagator.TILES) if name.split("x")[1].startswith("64"))
+
+
 def make(c, nx, ny, nup, U, precision=0, nw=16, seed=5, t2=0.0, blocks=True):
     model = HubbardModel.square(nx, ny, 2 * nup, U, 1.0, t2)
     phi = spin_block_trial(model, nup, nup) if blocks else free_electron_trial(model)
@@ -52,9 +56,9 @@ def block_of_work(prop, ws):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_every_tile_shape_gives_the_same_step(precision):
     ref = None
-    for t in range(6):
+    for t in range(len(CPPropagator.TILES)):
         prop, ws = make(ctx(), 4, 4, 5, 4.0, precision=precision)
-        prop.set_tiles(propagate=t, overlap=t, green=t if t in (0, 1, 4, 5) else 0)
+        prop.set_tiles(propagate=t, overlap=t, green=t if t in GREEN_TILES else 0)
         for _ in range(2):
             prop.step(ws)
         prop.measure(ws)
@@ -75,8 +79,10 @@ def test_autotune_keeps_state_and_results():
     after = ws_a.download()
     for x, y in zip(before, after):
         assert np.array_equal(x, y)
-    assert tiles.shape == (3,) and all(0 <= t < 6 for t in tiles) and tiles[2] in (0, 1, 4, 5)
-    assert np.all(ms[:, 0] > 0) and ms[2, 2] == -1.0 and ms[2, 3] == -1.0
+    nt = len(CPPropagator.TILES)
+    assert tiles.shape == (3,) and all(0 <= t < nt for t in tiles) and tiles[2] in GREEN_TILES
+    assert ms.shape == (3, nt) and np.all(ms[:, 0] > 0)
+    assert all(ms[2, t] == -1.0 for t in range(nt) if t not in GREEN_TILES)
     for _ in range(3):
         prop_a.step(ws_a)
         prop_b.step(ws_b)

@@ -39,6 +39,44 @@ def test_zgemm_real_a_matches_numpy(precision, shape):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("tile", [0, 3, 5])
+@pytest.mark.parametrize("N", [40, 112, 130])
+def test_zgemm_hermitian_gram(precision, tile, N):
+    """W^H W with only the tiles on or above the diagonal computed: those equal the full product's
+    bit for bit, the others are their exact conjugate transposes."""
+    rng = np.random.default_rng(N + tile + 11 * precision)
+    batch, K = 3, 96
+    W = crand(rng, batch, K, N)
+    full = zgemm(2, W, W, precision, tile=tile)
+    herm = zgemm(2, W, W, precision, herm=True, tile=tile)
+    T = 32 if tile == 3 else 64
+    for b in range(batch):
+        assert np.max(np.abs(herm[b] - W[b].conj().T @ W[b])) < TOL[precision] * np.sqrt(K) * 4
+        for i in range(N):
+            for j in range(N):
+                if i // T <= j // T:
+                    assert herm[b][i, j] == full[b][i, j]
+                else:
+                    assert herm[b][i, j] == np.conj(herm[b][j, i])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("tile", [0, 2, 3, 6])
+def test_zgemm_upper_triangular_b(precision, tile):
+    """W R^-1 with R^-1 upper triangular: the K tiles below an output column tile are skipped,
+    which leaves the result unchanged bit for bit."""
+    rng = np.random.default_rng(5 + tile + precision)
+    batch, M, N = 4, 150, 112
+    A = crand(rng, batch, M, N)
+    B = np.triu(crand(rng, batch, N, N))
+    ref = zgemm(0, A, B, precision, tile=tile)
+    tri = zgemm(0, A, B, precision, b_upper=True, tile=tile)
+    assert np.array_equal(ref, tri)
+    for b in range(batch):
+        assert np.max(np.abs(tri[b] - A[b] @ B[b])) < TOL[precision] * np.sqrt(N) * 4
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_zgemm_batched_strided_and_scaled(precision):
     rng = np.random.default_rng(3)
     batch, M, N, K = 5, 33, 20, 40
