@@ -58,9 +58,13 @@ def build(verbose=True, jobs=None):
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
     # C++ host driver on the C-ABI (plain g++: host code only, linked against the library)
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+    # (hipSOLVER for the one-time eigendecomposition of K when it reads a reference run directory)
+    rocm = os.path.dirname(os.path.dirname(HIPCC))
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(HERE, "..", "include"), "-I", os.path.join(rocm, "include"),
            os.path.join(HOST, "cp_run.cpp"), "-o", CP_RUN + ".tmp", "-L", HERE, "-lafqmc_mi355x",
-           "-Wl,-rpath,$ORIGIN"]
+           "-L", os.path.join(rocm, "lib"), "-lhipsolver", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"host driver build failed:\n{r.stderr}")

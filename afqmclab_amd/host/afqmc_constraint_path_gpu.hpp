@@ -5,17 +5,22 @@
 // afqmcConstraintPathMethod.h:15-45): run() -> initialWalker -> estimateMemory ->
 // measureWithProjection, the same projectOneStep / modifyGM / popControl / addMixedMeasurement /
 // adjustETAndResetMeasurement cadences, with the per-walker loops replaced by one batched
-// device call each.  Walkers, weights, ET and the random counter stay in HBM.
+// device call each.  Walkers, weights, ET and the random counter stay in HBM.  With an output
+// directory set, each block appends the reference's measurement files (den.dat, HNum.dat, ...,
+// HubbardSOCMeasureFixedSDSD.cpp:118-136) and beta.dat.
 #pragma once
+#include <cmath>
 #include <complex>
 #include <cstdio>
 #include <fstream>
+#include <iomanip>
 #include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/afqmc_mi355x.h"
+#include "reference_io.hpp"
 
 namespace afqmc_gpu {
 
@@ -23,51 +28,55 @@ inline void check(int rc, const char* what) {
   if (rc != 0) throw std::runtime_error(std::string(what) + ": " + afqmc_last_error());
 }
 
-// AfqmcConstraintPathMethod's keys (afqmcConstraintPathMethod.cpp:19-46), read from an
-// afqmc_param file of "key value" lines as readBySearchString does.
+// AfqmcConstraintPathMethod's keys and defaults (afqmcConstraintPathMethod.cpp:19-61), read from an
+// afqmc_param file as readBySearchString does: the value follows the key's token, "=" optional.
 struct ConstraintPathMethod {
   double dt = 0.01;
   size_t thermalSize = 200, writeNumber = 60, measureNumberPerWrite = 2, measureSkipStep = 5;
   int walkerSizePerThread = 300;
   std::string decompType = "densitySpin", forceType = "dynamicForce";
   double forceCap = 1.5;
+  std::string initialPhiTFlag = "setFromModel", initialWalkerFlag = "sampleFromPhiT";
   size_t mgsStep = 10, popControlStep = 10;
   double ET = -10.0;
   size_t ETAdjustStep = 10, ETAdjustMaxSize = 100;
   long long seed = 985456376;
 
-  template <typename T>
-  static bool find(const std::string& file, const std::string& key, T& value) {
-    std::ifstream in(file);
-    if (!in) throw std::runtime_error("cannot open " + file);
-    std::string line;
-    while (std::getline(in, line)) {
-      std::istringstream ss(line);
-      std::string k;
-      if (ss >> k && k == key) {
-        ss >> value;
-        return true;
-      }
-    }
-    return false;
-  }
   void read(const std::string& file) {
-    find(file, "dt", dt);
-    find(file, "thermalSize", thermalSize);
-    find(file, "writeNumber", writeNumber);
-    find(file, "measureNumberPerWrite", measureNumberPerWrite);
-    find(file, "measureSkipStep", measureSkipStep);
-    find(file, "walkerSizePerThread", walkerSizePerThread);
-    find(file, "decompType", decompType);
-    find(file, "forceType", forceType);
-    find(file, "forceCap", forceCap);
-    find(file, "mgsStep", mgsStep);
-    find(file, "popControlStep", popControlStep);
-    find(file, "ET", ET);
-    find(file, "ETAdjustStep", ETAdjustStep);
-    find(file, "ETAdjustMaxSize", ETAdjustMaxSize);
-    find(file, "seed", seed);
+    using refio::search_value;
+    search_value(file, "dt", dt);
+    search_value(file, "thermalSize", thermalSize);
+    search_value(file, "writeNumber", writeNumber);
+    search_value(file, "measureNumberPerWrite", measureNumberPerWrite);
+    search_value(file, "measureSkipStep", measureSkipStep);
+    search_value(file, "walkerSizePerThread", walkerSizePerThread);
+    search_value(file, "decompType", decompType);
+    search_value(file, "forceType", forceType);
+    search_value(file, "forceCap", forceCap);
+    search_value(file, "initialPhiTFlag", initialPhiTFlag);
+    search_value(file, "initialWalkerFlag", initialWalkerFlag);
+    search_value(file, "mgsStep", mgsStep);
+    search_value(file, "popControlStep", popControlStep);
+    search_value(file, "ET", ET);
+    search_value(file, "ETAdjustStep", ETAdjustStep);
+    search_value(file, "ETAdjustMaxSize", ETAdjustMaxSize);
+    search_value(file, "seed", seed);
     analysis();
+  }
+  // afqmcConstraintPathMethod.cpp:48-89
+  void write(const std::string& file) const {
+    std::ofstream f(file, std::ios::out | std::ios::trunc);
+    if (!f) throw std::runtime_error("Error opening file in File!!! " + file);
+    auto kv = [&](const char* k, const auto& v) { f << std::setw(36) << (std::string(k) + " ") << std::setw(26) << v << "\n"; };
+    f << std::left << "\n";
+    kv("dt", dt); kv("thermalSize", thermalSize); kv("writeNumber", writeNumber);
+    kv("measureNumberPerWrite", measureNumberPerWrite); kv("measureSkipStep", measureSkipStep); f << "\n";
+    kv("walkerSizePerThread", walkerSizePerThread); f << "\n";
+    kv("decompType", decompType); kv("forceType", forceType); kv("forceCap", forceCap);
+    kv("initialPhiTFlag", initialPhiTFlag); kv("initialWalkerFlag", initialWalkerFlag); f << "\n";
+    kv("mgsStep", mgsStep); kv("popControlStep", popControlStep); f << "\n";
+    kv("ET", ET); kv("ETAdjustStep", ETAdjustStep); kv("ETAdjustMaxSize", ETAdjustMaxSize); f << "\n";
+    kv("seed", seed); f << "\n";
   }
   // afqmcConstraintPathMethod.cpp:182-229, as errors instead of exit(1)
   void analysis() const {
@@ -140,23 +149,52 @@ class AfqmcConstraintPathGpu {
   void run() {
     initialWalker();
     estimateMemory();
-    measureWithProjection();
+    if (std::abs(method.dt) < 1e-12) measureWithoutProjection();
+    else measureWithProjection();
+    prepareStop();
   }
   const std::vector<BlockEnergy>& energies() const { return blocks; }
+  const ConstraintPathMethod& getMethod() const { return method; }
+
+  // Initial walkers other than copies of phiT (initialWalkerFlag readFromFile, readAllWalkers,
+  // setFromModel): nw walkers of 2L x N (column-major each), their log weights and alive flags.
+  void setInitialWalkers(std::vector<std::complex<double>> wf, std::vector<std::complex<double>> logw,
+                         std::vector<int> alive) {
+    const size_t n = (size_t)2 * model.L * model.N, nw = (size_t)method.walkerSizePerThread;
+    if (wf.size() != n * nw || logw.size() != nw || alive.size() != nw)
+      throw std::runtime_error("setInitialWalkers: expected " + std::to_string(nw) + " walkers of 2L x N");
+    initWf = std::move(wf);
+    initLogw = std::move(logw);
+    initAlive = std::move(alive);
+  }
+  // Blocks append the reference's measurement files and beta.dat in this directory.
+  void setOutputDirectory(const std::string& dir) { outDir = dir; }
+  // The walkers after prepareStop (projected back by exp(dt K / 2)), as writeWalkers sees them.
+  void finalWalkers(std::vector<std::complex<double>>& wf, std::vector<std::complex<double>>& logw,
+                    std::vector<int>& alive) const {
+    wf = finalWf;
+    logw = finalLogw;
+    alive = finalAlive;
+  }
   uint64_t counter() const {
     uint64_t c = 0;
     check(afqmc_cp_get_counter(cp, &c), "afqmc_cp_get_counter");
     return c;
   }
 
-  // initialWalkerFlag = sampleFromPhiT: every walker = phiT, then QR and comb (initialMgsAndPopControl)
+  // initialWalkerFlag = sampleFromPhiT: every walker = phiT (or the walkers given by
+  // setInitialWalkers), then QR and comb (initialMgsAndPopControl, afqmcConstraintPathSetWalker.cpp)
   void initialWalker() {
-    const int nw = method.walkerSizePerThread;
-    const size_t n = model.phiT.size();
-    std::vector<std::complex<double>> wf(n * nw);
-    for (int i = 0; i < nw; ++i) std::copy(model.phiT.begin(), model.phiT.end(), wf.begin() + n * i);
-    std::vector<std::complex<double>> logw(nw, 0.0);
-    check(afqmc_walkers_upload(walker, dp(wf), dp(logw), nullptr), "afqmc_walkers_upload");
+    if (!initWf.empty()) {
+      check(afqmc_walkers_upload(walker, dp(initWf), dp(initLogw), initAlive.data()), "afqmc_walkers_upload");
+    } else {
+      const int nw = method.walkerSizePerThread;
+      const size_t n = model.phiT.size();
+      std::vector<std::complex<double>> wf(n * nw);
+      for (int i = 0; i < nw; ++i) std::copy(model.phiT.begin(), model.phiT.end(), wf.begin() + n * i);
+      std::vector<std::complex<double>> logw(nw, 0.0);
+      check(afqmc_walkers_upload(walker, dp(wf), dp(logw), nullptr), "afqmc_walkers_upload");
+    }
     modifyGM();
     popControl();
   }
@@ -179,8 +217,32 @@ class AfqmcConstraintPathGpu {
         addMixedMeasurement();
         for (size_t k = 0; k < method.measureSkipStep; ++k) projectOneStep(mgsIndex, popControlIndex);
       }
+      if (!outDir.empty()) {
+        const double beta = (method.thermalSize + (i + 0.5) * method.measureNumberPerWrite * method.measureSkipStep - 0.5) *
+                            method.dt;
+        refio::append(outDir + "/beta.dat", beta);
+      }
       writeAndResetMeasurement();
     }
+  }
+  // afqmcConstraintPath.cpp:89-96 (dt = 0)
+  void measureWithoutProjection() {
+    projectExpMinusHalfDtK();
+    addMixedMeasurement();
+    writeAndResetMeasurement();
+  }
+  // afqmcConstraintPath.cpp:149-155: the adjusted ET goes back into the method, the walkers are
+  // projected back by exp(dt K / 2) and kept for writeWalkers
+  void prepareStop() {
+    check(afqmc_cp_get_ET(cp, &method.ET), "afqmc_cp_get_ET");
+    check(afqmc_cp_onebody(cp, walker, 2), "afqmc_cp_onebody");
+    const size_t n = (size_t)2 * model.L * model.N, nw = (size_t)method.walkerSizePerThread;
+    finalWf.assign(n * nw, 0.0);
+    finalLogw.assign(nw, 0.0);
+    finalAlive.assign(nw, 0);
+    check(afqmc_walkers_download(walker, reinterpret_cast<double*>(finalWf.data()),
+                                 reinterpret_cast<double*>(finalLogw.data()), finalAlive.data()),
+          "afqmc_walkers_download");
   }
 
  private:
@@ -199,6 +261,29 @@ class AfqmcConstraintPathGpu {
     double e[14];
     check(afqmc_cp_energy(cp, e), "afqmc_cp_energy");
     blocks.push_back({{e[2], e[3]}, {e[0], e[1]}, {e[4], e[5]}, {e[6], e[7]}});
+    if (!outDir.empty()) {
+      using cd = std::complex<double>;
+      const size_t L = (size_t)model.L;
+      std::vector<cd> site(4 * L);
+      check(afqmc_cp_site_measure(cp, reinterpret_cast<double*>(site.data())), "afqmc_cp_site_measure");
+      auto c = [&](int k) { return cd(e[2 * k], e[2 * k + 1]); };
+      auto part = [&](int q) { return std::vector<cd>(site.begin() + q * L, site.begin() + (q + 1) * L); };
+      cd splusTot = 0.0, sminusTot = 0.0;
+      for (size_t i = 0; i < L; ++i) { splusTot += site[2 * L + i]; sminusTot += site[3 * L + i]; }
+      refio::append(outDir + "/den.dat", c(0));
+      refio::append(outDir + "/KNum.dat", c(2));
+      refio::append(outDir + "/VNum.dat", c(3));
+      refio::append(outDir + "/RNum.dat", c(4));
+      refio::append(outDir + "/HNum.dat", c(1));
+      refio::append(outDir + "/NupNum.dat", part(0));
+      refio::append(outDir + "/NdnNum.dat", part(1));
+      refio::append(outDir + "/SplusNum.dat", part(2));
+      refio::append(outDir + "/SminusNum.dat", part(3));
+      refio::append(outDir + "/NupTotNum.dat", c(5));
+      refio::append(outDir + "/NdnTotNum.dat", c(6));
+      refio::append(outDir + "/SplusTotNum.dat", splusTot);
+      refio::append(outDir + "/SminusTotNum.dat", sminusTot);
+    }
     check(afqmc_cp_reset_measure(cp), "afqmc_cp_reset_measure");
   }
 
@@ -212,6 +297,9 @@ class AfqmcConstraintPathGpu {
   afqmc_cp cp = nullptr;
   afqmc_walkers walker = nullptr;
   std::vector<BlockEnergy> blocks;
+  std::vector<std::complex<double>> initWf, initLogw, finalWf, finalLogw;
+  std::vector<int> initAlive, finalAlive;
+  std::string outDir;
 };
 
 }  // namespace afqmc_gpu

@@ -1,0 +1,215 @@
+"""Files in the formats of the reference's constraint-path tutorial, and its Monte Carlo analysis.
+
+A run directory prepared for the reference (``model_param``, ``afqmc_param``, ``phiT.dat``,
+``phi.dat``, ``walkers/phi_<i>.dat``) can be read, and one can be written for
+``afqmc_cp_run``; the measurement files a run appends (``HNum.dat``, ``den.dat``, ...) are read and
+analysed as ``NumDenErrorAnalysis`` does.
+
+Formats:
+
+* whitespace-separated values, a complex number as "re im" (readWriteFstream.h:12-60), written one
+  per line in 16-digit scientific notation, width 26;
+* TensorHao: the rank, the dimensions, then the elements column-major (tensor_core.h:278-300);
+* SD: logw, then its 2-D TensorHao (SD.cpp:85-103);
+* HubbardSOC ``model_param``: L, N, K (2L x 2L complex), mu, hx, hy, hz, U (HubbardSOC.cpp:39-74);
+* ``afqmc_param``: "key value" pairs found as whitespace tokens, with an optional "=" between
+  them (readBySearchString.cpp:16-60), written as afqmcConstraintPathMethod.cpp:48-89 writes them.
+"""
+import numpy as np
+
+from .model import HubbardModel
+
+_FMT = "{:26.16e}"
+
+
+def _tokens(path):
+    with open(path) as f:
+        return f.read().split()
+
+
+def _fmt_real(x):
+    return _FMT.format(float(x)) + "\n"
+
+
+def _fmt_complex(z):
+    z = complex(z)
+    return _FMT.format(z.real) + _FMT.format(z.imag) + "\n"
+
+
+class _Reader:
+    def __init__(self, path):
+        self.path = path
+        self.t = _tokens(path)
+        self.i = 0
+
+    def real(self):
+        if self.i >= len(self.t):
+            raise ValueError(f"{self.path}: unexpected end of file")
+        v = float(self.t[self.i])
+        self.i += 1
+        return v
+
+    def int(self):
+        if self.i >= len(self.t):
+            raise ValueError(f"{self.path}: unexpected end of file")
+        v = int(self.t[self.i])
+        self.i += 1
+        return v
+
+    def reals(self, n):
+        return np.array([self.real() for _ in range(n)])
+
+    def complexes(self, n):
+        v = self.reals(2 * n)
+        return v[0::2] + 1j * v[1::2]
+
+
+def read_model_param(path):
+    """HubbardSOC::read: a HubbardModel."""
+    r = _Reader(path)
+    L = r.int()
+    N = r.int()
+    K = r.complexes(4 * L * L).reshape(2 * L, 2 * L, order="F")
+    mu, hx, hy, hz, U = (r.reals(L) for _ in range(5))
+    return HubbardModel(L=L, N=N, K=K, U=U, mu=mu, hx=hx, hy=hy, hz=hz)
+
+
+def write_model_param(path, model):
+    """HubbardSOC::write."""
+    with open(path, "w") as f:
+        f.write(f"{model.L:26d}\n{model.N:26d}\n")
+        f.writelines(_fmt_complex(z) for z in np.asarray(model.K).ravel(order="F"))
+        for v in (model.mu, model.hx, model.hy, model.hz, model.U):
+            f.writelines(_fmt_real(x) for x in np.asarray(v))
+
+
+def read_sd(path):
+    """SD::read: (logw, wf) with wf a rows x cols complex array."""
+    r = _Reader(path)
+    logw = complex(*r.reals(2))
+    rank = r.int()
+    if rank != 2:
+        raise ValueError(f"{path}: Dimen is not consistent! 2 {rank}")
+    rows, cols = r.int(), r.int()
+    wf = r.complexes(rows * cols).reshape(rows, cols, order="F")
+    return logw, wf
+
+
+def write_sd(path, logw, wf):
+    """SD::write."""
+    wf = np.asarray(wf, dtype=np.complex128)
+    with open(path, "w") as f:
+        f.write(_fmt_complex(logw))
+        f.write(f"2\n{wf.shape[0]} {wf.shape[1]} \n")
+        f.writelines(_fmt_complex(z) for z in wf.ravel(order="F"))
+
+
+def read_values(path, skip=0):
+    """A measurement file written with ios::app: one complex value per line, after `skip` lines."""
+    v = np.loadtxt(path, ndmin=2)
+    z = v[:, 0] + 1j * v[:, 1]
+    return z[skip:]
+
+
+# afqmcConstraintPathMethod.cpp:48-89: the keys in the order the reference writes them
+METHOD_KEYS = [("dt", "thermalSize", "writeNumber", "measureNumberPerWrite", "measureSkipStep"),
+               ("walkerSizePerThread",),
+               ("decompType", "forceType", "forceCap", "initialPhiTFlag", "initialWalkerFlag"),
+               ("mgsStep", "popControlStep"),
+               ("ET", "ETAdjustStep", "ETAdjustMaxSize"),
+               ("seed",)]
+
+
+def write_afqmc_param(path, values):
+    """An afqmc_param file from a dict of the method keys (missing keys are left out; the reader
+    then keeps its defaults)."""
+    with open(path, "w") as f:
+        f.write("\n")
+        for group in METHOD_KEYS:
+            for k in group:
+                if k in values:
+                    f.write(f"{k + ' ':<36}{values[k]!s:<26}\n")
+            f.write("\n")
+
+
+def search_value(path, key, cast=str):
+    """readBySearchString: the value after the token `key` ("=" optional), or None."""
+    t = _tokens(path)
+    for i, tok in enumerate(t):
+        if tok == key and i + 1 < len(t):
+            v = t[i + 1]
+            if v == "=" and i + 2 < len(t):
+                v = t[i + 2]
+            return cast(v)
+    return None
+
+
+# ------------------------------------------------------------------ NumDenErrorAnalysis
+def get_divisors(n):
+    """serialErrorAnalysis.cpp:8-26: the divisors of n, ascending."""
+    out = set()
+    i = 1
+    while i * i <= n:
+        if n % i == 0:
+            out.add(i)
+            out.add(n // i)
+        i += 1
+    return sorted(out)
+
+
+def get_mean_error(a):
+    """serialErrorAnalysis.h:18-46: the mean and its standard error sqrt(var / (n - 1)), var the
+    mean of |a - mean|^2 (0 for one sample)."""
+    a = np.asarray(a)
+    mean = a.mean()
+    if a.size == 1:
+        return mean, 0.0
+    var = np.mean(np.abs(a - mean) ** 2)
+    return mean, float(np.sqrt(var / (a.size - 1)))
+
+
+def num_den_error_analysis(num, den, skip=0):
+    """NumDenErrorAnalysis.cpp:12-64: for every divisor b of the sample count, the ratio of block
+    sums sum(num)/sum(den) over blocks of b samples, its mean and error.  Returns rows
+    (b, mean_re, mean_im, err)."""
+    num = np.asarray(num)[skip:]
+    den = np.asarray(den)[skip:]
+    n = min(num.size, den.size)
+    rows = []
+    for b in get_divisors(n):
+        nb = n // b
+        r = num[:nb * b].reshape(nb, b).sum(axis=1) / den[:nb * b].reshape(nb, b).sum(axis=1)
+        mean, err = get_mean_error(r)
+        rows.append((b, mean.real, mean.imag, err))
+    return rows
+
+
+def write_num_den_analysis(out_path, rows):
+    with open(out_path, "w") as f:
+        for b, mr, mi, e in rows:
+            f.write(f"{b:26d}" + _FMT.format(mr) + _FMT.format(mi) + _FMT.format(e) + "\n")
+
+
+def main(argv=None):
+    import sys
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) < 3:
+        print("usage: python -m afqmclab_amd.reference_io num.dat den.dat out.dat [skipStep]")
+        return 1
+    skip = int(argv[3]) if len(argv) > 3 else 0
+    num = np.loadtxt(argv[0], ndmin=2)
+    den = np.loadtxt(argv[1], ndmin=2)
+    n = min(len(num), len(den)) - skip
+    print(f"Effective sample points is {n}")
+    rows = num_den_error_analysis(num[:, 0] + 1j * num[:, 1], den[:, 0] + 1j * den[:, 1], skip)
+    write_num_den_analysis(argv[2], rows)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
+
+
+__all__ = ["read_model_param", "write_model_param", "read_sd", "write_sd", "read_values", "write_afqmc_param",
+           "search_value", "get_divisors", "get_mean_error", "num_den_error_analysis", "write_num_den_analysis",
+           "METHOD_KEYS"]

@@ -63,4 +63,4 @@ def test_host_driver_reports_missing_input(tmp_path):
     import subprocess
     r = subprocess.run([EXE, str(tmp_path / "nothing")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
-    assert "cannot open" in r.stderr
+    assert "Error opening file" in r.stderr and "afqmc_param" in r.stderr

@@ -71,3 +71,66 @@ def test_cpp_driver_matches_python_driver_and_oracle(tmp_path, mode):
     ref.run()
     for h, d, e in zip(H, D, ref.energies):
         assert abs(h / d - e[0] / e[1]) < 1e-9 * abs(e[0] / e[1])
+
+
+def run_reference_dir(d):
+    r = subprocess.run([EXE, str(d)], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+@pytest.mark.parametrize("phi_flag", ["readFromFile", "setFromModel"])
+def test_cpp_driver_runs_a_reference_run_directory(tmp_path, phi_flag):
+    """afqmc_cp_run on a directory in the reference's formats (model_param, afqmc_param, phiT.dat):
+    exp(-alpha K) and the setFromModel trial from K's eigendecomposition, the measurement files,
+    beta.dat, afqmc_param with the adjusted ET and walkers/ written as the reference writes them,
+    the same block energies as the Python driver on the same trial; then a restart from the
+    written walkers (readAllWalkers)."""
+    from afqmclab_amd import reference_io as rio
+    nup = ndn = 5
+    model = HubbardModel.square(4, 4, nup + ndn, 4.0)
+    m = dict(dt=0.01, thermalSize=20, writeNumber=3, measureNumberPerWrite=2, measureSkipStep=5,
+             walkerSizePerThread=16, decompType="densitySpin", forceType="dynamicForce", forceCap=1.5,
+             initialPhiTFlag=phi_flag, initialWalkerFlag="sampleFromPhiT", mgsStep=10, popControlStep=10,
+             ET=-10.0, ETAdjustStep=10, ETAdjustMaxSize=20, seed=31)
+    rio.write_afqmc_param(tmp_path / "afqmc_param", m)
+    rio.write_model_param(tmp_path / "model_param", model)
+    if phi_flag == "readFromFile":
+        rio.write_sd(tmp_path / "phiT.dat", 0.0, spin_block_trial(model, nup, ndn))
+    print(f"[cpp driver] reference directory, {phi_flag}", flush=True)
+    out = run_reference_dir(tmp_path)
+    _, phi = rio.read_sd(tmp_path / "phiT.dat")
+    blocks = phi_flag == "readFromFile"
+    assert f"spinBlocks {int(blocks)}" in out
+    H, D = rio.read_values(tmp_path / "HNum.dat"), rio.read_values(tmp_path / "den.dat")
+    K, V = rio.read_values(tmp_path / "KNum.dat"), rio.read_values(tmp_path / "VNum.dat")
+    assert len(H) == len(D) == len(K) == len(V) == m["writeNumber"]
+    assert np.allclose(H, K + V, rtol=1e-12)
+    nupn = rio.read_values(tmp_path / "NupNum.dat")
+    assert len(nupn) == m["writeNumber"] * model.L
+    ntot = rio.read_values(tmp_path / "NupTotNum.dat")
+    assert np.allclose(ntot, nupn.reshape(m["writeNumber"], model.L).sum(axis=1), rtol=1e-12)
+    beta = np.loadtxt(tmp_path / "beta.dat")
+    assert np.allclose(beta, [(20 + (i + 0.5) * 10 - 0.5) * 0.01 for i in range(3)])
+    kw = dict(nup=nup, spin_blocks=True) if blocks else dict(spin_blocks=False)
+    mm = {k: v for k, v in m.items() if k not in ("initialPhiTFlag", "initialWalkerFlag")}
+    dev = AfqmcConstraintPath(ctx(), model, CPMethod(**mm), phi, **kw)
+    dev.run()
+    for h, d, e in zip(H, D, dev.energies):
+        assert abs(h / d - e[0] / e[1]) <= 1e-9 * abs(e[0] / e[1])
+    assert rio.search_value(tmp_path / "afqmc_param", "ET", float) != -10.0
+    files = sorted(os.listdir(tmp_path / "walkers"))
+    assert 0 < len(files) <= 16
+    logw, wf = rio.read_sd(tmp_path / "walkers" / files[0])
+    assert wf.shape == (2 * model.L, nup + ndn)
+    # restart from the written walkers
+    for f in ("HNum.dat", "den.dat"):
+        os.remove(tmp_path / f)
+    m2 = dict(m, initialPhiTFlag="readFromFile", initialWalkerFlag="readAllWalkers", thermalSize=10, writeNumber=2,
+              ETAdjustMaxSize=10)
+    rio.write_afqmc_param(tmp_path / "afqmc_param", m2)
+    run_reference_dir(tmp_path)
+    H2 = rio.read_values(tmp_path / "HNum.dat")
+    D2 = rio.read_values(tmp_path / "den.dat")
+    assert len(H2) == 2 and np.all(np.isfinite(H2 / D2))
+    assert abs((H2 / D2).real.mean() - (H / D).real.mean()) < 0.05 * abs((H / D).real.mean())
