@@ -19,8 +19,8 @@ import numpy as np
 
 from . import comb
 from .blocks import (SD, SD2is, HopSDOperation, LogHop2isSD2isOperation, SDSDOperation,
-                     SD2sSD2isOperation, niupnidn_apply_right)
-from .models import (HubbardSOCMeasureFixedSDSD, RealMaterialMoleculeMeasureFixedSD2sSD2is,
+                     SD2sSD2isOperation, niupnidn_apply_left, niupnidn_apply_right)
+from .models import (HubbardSOCMeasureFixedSDSD, HubbardSOCMeasureObserveSDSD, RealMaterialMoleculeMeasureFixedSD2sSD2is,
                      fillWalkerFromModel, fillSD2sFromModel, sd2is_from_phiT)
 from .rng import gaussians_from_uniforms
 
@@ -205,6 +205,191 @@ class AfqmcConstraintPath:
         self.estimateMemory()
         self.measureWithProjection()
         return self.energies
+
+
+# --------------------------------------------------------------------------
+# Constraint path with back-propagation
+# --------------------------------------------------------------------------
+@dataclass
+class BPMethod(CPMethod):
+    """afqmc_param keys of the back-propagation driver: the constraint-path keys and
+    backPropagationStep (tutorials/afqmcConstraintPathBP/source/afqmcConstraintPathMethod.cpp:19-46,
+    default 100 at :166)."""
+    backPropagationStep: int = 100
+
+
+class AfqmcConstraintPathBP(AfqmcConstraintPath):
+    """Serial back-propagation driver (tutorials/afqmcConstraintPathBP/source/*.cpp).
+
+    The step is the constraint-path one; while isBP, each alive walker's aux field of the step is
+    recorded in twoBodyAuxBackup[i][BPIndex] and each comb table in tablePerThreadBackup[BPIndex]
+    (afqmcConstraintPathProjection.cpp:73-77,163-171).  A measurement replays the recorded tables on
+    (aux history, backup walker) pairs, rebuilds the next window's backup by propagating the first
+    measureSkipStep recorded fields (afqmcConstraintPathBackPropagation.cpp:22-97), and measures
+    <bra_i| O |ket_i> with bra_i = phiT propagated to the left through walker i's recorded fields
+    and ket_i = its backup walker, weighted by the walker's mixed overlap
+    (afqmcConstraintPathMeasure.cpp:41-72).  Observables are HubbardSOCMeasureObserveSDSD's."""
+
+    def __init__(self, method: BPMethod, model, rng, phiT: SD = None, constForce=None):
+        super().__init__(method, model, rng, phiT, constForce)
+        n, nbp = method.walkerSizePerThread, method.backPropagationStep
+        self.observeMeasure = HubbardSOCMeasureObserveSDSD(model)
+        self.isBP = False
+        self.BPIndex = 0
+        self.walkerBackup = [None] * n
+        self.walkerBPMeasure = [None] * n
+        self.twoBodyAuxBackup = [[None] * nbp for _ in range(n)]
+        self.twoBodyAuxBPMeasure = [[None] * nbp for _ in range(n)]
+        self.tablePerThreadBackup = [None] * nbp
+        self.writes = []
+        self.betas = []
+
+    # -- recording (afqmcConstraintPathProjection.cpp) --------------------------
+    def step_walker(self, i):
+        before = len(self.records)
+        super().step_walker(i)
+        if self.isBP and len(self.records) > before:
+            self.twoBodyAuxBackup[i][self.BPIndex] = self.records[-1][2].copy()
+
+    def projectExpMinusDtKExpMinusDtV(self):
+        super().projectExpMinusDtKExpMinusDtV()
+        if self.isBP:
+            self.tablePerThreadBackup[self.BPIndex] = None
+
+    def popControl(self):
+        weight, table = super().popControl()
+        if self.isBP:
+            self.tablePerThreadBackup[self.BPIndex] = table.copy()
+        return weight, table
+
+    def projectOneStep(self, idx):
+        super().projectOneStep(idx)
+        if self.isBP:
+            self.BPIndex += 1
+
+    # -- afqmcConstraintPathBackPropagation.cpp -------------------------------
+    def prepareBackPropagation(self):
+        self.isBP = True
+        self.BPIndex = 0
+        self.walkerBackup = [w.copy() for w in self.walker]
+
+    def _apply_table_to_records(self, table, nrec):
+        """populationControl over AfqmcBackPropagationPop: slot i takes slot table[i]'s first
+        nrec aux fields and its backup walker (afqmcBackPropagationPop.cpp:26-34)."""
+        for i in range(len(table)):
+            j = int(table[i])
+            if i != j:
+                for k in range(nrec):
+                    self.twoBodyAuxBPMeasure[i][k] = self.twoBodyAuxBPMeasure[j][k]
+                self.walkerBPMeasure[i] = self.walkerBPMeasure[j].copy()
+
+    def prepareBackPropagationMeasAndNextBackup(self):
+        m = self.method
+        nbp, skip = m.backPropagationStep, m.measureSkipStep
+        self.walkerBPMeasure, self.walkerBackup = self.walkerBackup, self.walkerBPMeasure
+        if nbp >= skip:
+            self.twoBodyAuxBPMeasure = [list(h) for h in self.twoBodyAuxBackup]
+        else:
+            self.twoBodyAuxBPMeasure, self.twoBodyAuxBackup = self.twoBodyAuxBackup, self.twoBodyAuxBPMeasure
+        for i in range(nbp):
+            if self.tablePerThreadBackup[i] is not None:
+                self._apply_table_to_records(self.tablePerThreadBackup[i], i + 1)
+            if i + 1 == skip:
+                self.generateWalkerBackupFromWalkerBPMeasure()
+        if nbp >= skip:
+            self.BPIndex = nbp - skip
+            for i in range(nbp - skip):
+                self.tablePerThreadBackup[i] = self.tablePerThreadBackup[i + skip]
+                for j in range(len(self.walker)):
+                    self.twoBodyAuxBackup[j][i] = self.twoBodyAuxBackup[j][i + skip]
+        else:
+            self.isBP = False
+
+    def generateWalkerBackupFromWalkerBPMeasure(self):
+        m = self.method
+        for i in range(len(self.walker)):
+            w = self.walkerBPMeasure[i].copy()
+            for j in range(m.measureSkipStep):
+                aux = self.twoBodyAuxBPMeasure[i][j]
+                if aux is not None and aux.size == self.model.L:
+                    sample = self.expMinusDtV.getTwoBodySampleFromAux(aux)
+                    wf, lw = niupnidn_apply_right(sample, w.wf, w.logw)
+                    w = SD(wf=self.expMinusDtK.matrix @ wf, logw=self.expMinusDtK.logw + lw)
+                if j % m.mgsStep == 0:
+                    w.normalize()
+            self.walkerBackup[i] = w
+
+    # -- afqmcConstraintPathMeasure.cpp ---------------------------------------
+    def addMixedMeasurement(self):
+        for i, w in enumerate(self.walker):
+            if self.walkerIsAlive[i]:
+                t = self.op1.applyToRight(self.expHalfDtK, w)
+                op = SDSDOperation(self.phiT, t)
+                self.observeMeasure.addMeasurement(op, cmath.exp(op.returnLogOverlap()))
+
+    def back_propagated_bra(self, history):
+        """phiT propagated to the left through one recorded history (afqmcConstraintPathMeasure.cpp:56-65)."""
+        m = self.method
+        nbp = m.backPropagationStep
+        bra = self.op1.applyToLeft(self.expMinusHalfDtK, self.phiT)
+        for j in range(nbp, 0, -1):
+            aux = history[j - 1]
+            if aux is None or aux.size != self.model.L:
+                raise ValueError("Aux size does not consistent with L!")
+            sample = self.expMinusDtV.getTwoBodySampleFromAux(aux)
+            wf, lw = niupnidn_apply_left(sample, bra.wf, bra.logw)
+            bra = self.op1.applyToLeft(self.expMinusDtK, SD(wf=wf, logw=lw))
+            if (nbp - j + 1) % m.mgsStep == 0:
+                bra.normalize()
+        return self.op1.applyToLeft(self.expHalfDtK, bra)
+
+    def addMeasurement(self):
+        if not self.isBP:
+            return
+        self.prepareBackPropagationMeasAndNextBackup()
+        for i, w in enumerate(self.walker):
+            if self.walkerIsAlive[i]:
+                t = self.op1.applyToRight(self.expHalfDtK, w)
+                overlap = cmath.exp(SDSDOperation(self.phiT, t).returnLogOverlap())
+                bra = self.back_propagated_bra(self.twoBodyAuxBPMeasure[i])
+                ket = self.op1.applyToRight(self.expHalfDtK, self.walkerBPMeasure[i])
+                self.observeMeasure.addMeasurement(SDSDOperation(bra, ket), overlap)
+
+    def measureWithProjection(self):
+        """afqmcConstraintPath.cpp:115-180 of the back-propagation tutorial."""
+        m = self.method
+        nbp, skip = m.backPropagationStep, m.measureSkipStep
+        additional = nbp if nbp >= skip else 0
+        self.projectExpMinusHalfDtK()
+        idx = [0, 0]
+        for i in range(m.thermalSize):
+            if i < m.ETAdjustMaxSize and i % m.ETAdjustStep == 0:
+                self.addMixedMeasurement()
+                m.ET = self.observeMeasure.returnEnergy().real
+                self.observeMeasure.reSet()
+            self.projectOneStep(idx)
+        self.prepareBackPropagation()
+        if nbp >= skip:
+            for _ in range(nbp):
+                self.projectOneStep(idx)
+        for i in range(m.writeNumber):
+            for j in range(m.measureNumberPerWrite):
+                for k in range(skip):
+                    if self.BPIndex == nbp:
+                        self.addMeasurement()
+                    self.projectOneStep(idx)
+                if nbp < skip:
+                    self.prepareBackPropagation()
+            self.betas.append((m.thermalSize + (i + 0.5) * m.measureNumberPerWrite * skip - 0.5) * m.dt)
+            self.writes.append(self.observeMeasure.snapshot())
+            self.observeMeasure.reSet()
+        return additional
+
+    def run(self):
+        self.initialWalker()
+        self.estimateMemory()
+        self.measureWithProjection()
+        return self.writes
 
 
 # --------------------------------------------------------------------------

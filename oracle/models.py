@@ -194,6 +194,98 @@ class HubbardSOCMeasureFixedSDSD:
         return np.clip(f, -cap, cap)
 
 
+class HubbardSOCMeasureObserveSDSD:
+    """Observables of a general bra/ket pair from its full Green matrix G(i,j) = <c+_i c_j>
+    (HubbardSOCMeasureObserveSDSD.cpp:34-183 over HubbardSOCMeasureCommuteSDSD.cpp:45-180):
+    den, H/K/V/R, greenMatrixNum (2L x 2L), densityDensityNum (2L x 2L, <n_a n_b>),
+    splusSminusNum, sminusSplusNum, spairSpairNum (L x L), each summed with weight denIncrement.
+    The back-propagation driver measures through it.  Pinned by a Fock-space brute force of every
+    operator (tests/test_oracle_bp.py)."""
+
+    def __init__(self, model: HubbardSOC):
+        self.model = model
+        self.reSet()
+
+    def reSet(self):
+        L = self.model.L
+        self.den = 0j
+        self.HNum = 0j
+        self.KNum = 0j
+        self.VNum = 0j
+        self.RNum = 0j
+        self.greenMatrixNum = np.zeros((2 * L, 2 * L), dtype=C128)
+        self.densityDensityNum = np.zeros((2 * L, 2 * L), dtype=C128)
+        self.splusSminusNum = np.zeros((L, L), dtype=C128)
+        self.sminusSplusNum = np.zeros((L, L), dtype=C128)
+        self.spairSpairNum = np.zeros((L, L), dtype=C128)
+
+    def returnEnergy(self):
+        return self.HNum / self.den
+
+    def getForce(self, niupNidn: NiupNidn, op, cap):
+        """HubbardSOCMeasureCommuteSDSD.cpp:63-104: the same background as the fixed-trial force."""
+        return HubbardSOCMeasureFixedSDSD.getForce(self, niupNidn, op, cap)
+
+    @staticmethod
+    def energy_terms(model, G):
+        """(K, V, R) of addEnergy (HubbardSOCMeasureCommuteSDSD.cpp:143-180)."""
+        L = model.L
+        d = np.diag(G)
+        Ke = np.sum(model.K * G)
+        Ve = np.sum(model.U * (d[:L] * d[L:] - np.diag(G[:L, L:]) * np.diag(G[L:, :L])))
+        Re = (np.sum((-model.mu + model.hz * 0.5) * d[:L]) + np.sum((-model.mu - model.hz * 0.5) * d[L:])
+              + np.sum((model.hx * 0.5 - 0.5j * model.hy) * np.diag(G[:L, L:]))
+              + np.sum((model.hx * 0.5 + 0.5j * model.hy) * np.diag(G[L:, :L])))
+        return Ke, Ve, Re
+
+    @staticmethod
+    def correlators(G):
+        """densityDensity, splusSminus, sminusSplus, spairSpair of one Green matrix
+        (HubbardSOCMeasureObserveSDSD.cpp:77-183), written as loops over (i, j) as the reference."""
+        L2 = G.shape[0]
+        L = L2 // 2
+        dd = np.empty((L2, L2), dtype=C128)
+        for j in range(L2):
+            for i in range(L2):
+                dd[i, j] = G[i, i] if i == j else G[i, i] * G[j, j] - G[i, j] * G[j, i]
+        pm = np.empty((L, L), dtype=C128)
+        mp = np.empty((L, L), dtype=C128)
+        sp = np.empty((L, L), dtype=C128)
+        for j in range(L):
+            for i in range(L):
+                det = G[i, i] * G[i + L, i + L] - G[i, i + L] * G[i + L, i]
+                if i == j:
+                    pm[i, j] = G[i, i] - det
+                    mp[i, j] = G[i + L, i + L] - det
+                else:
+                    pm[i, j] = -G[i, j] * G[j + L, i + L] + G[i, i + L] * G[j + L, j]
+                    mp[i, j] = -G[i + L, j + L] * G[j, i] + G[i + L, i] * G[j, j + L]
+                sp[i, j] = G[i, j] * G[i + L, j + L] - G[i, j + L] * G[i + L, j]
+        return dd, pm, mp, sp
+
+    def addMeasurement(self, op, denIncrement):
+        G = op.returnGreenMatrix()
+        self.den += denIncrement
+        Ke, Ve, Re = self.energy_terms(self.model, G)
+        self.HNum += (Ke + Ve + Re) * denIncrement
+        self.KNum += Ke * denIncrement
+        self.VNum += Ve * denIncrement
+        self.RNum += Re * denIncrement
+        dd, pm, mp, sp = self.correlators(G)
+        self.greenMatrixNum += G * denIncrement
+        self.densityDensityNum += dd * denIncrement
+        self.splusSminusNum += pm * denIncrement
+        self.sminusSplusNum += mp * denIncrement
+        self.spairSpairNum += sp * denIncrement
+
+    def snapshot(self):
+        """The accumulators as write() puts them on disk (HubbardSOCMeasureObserveSDSD.cpp:47-57)."""
+        return dict(den=self.den, HNum=self.HNum, KNum=self.KNum, VNum=self.VNum, RNum=self.RNum,
+                    greenMatrixNum=self.greenMatrixNum.copy(), densityDensityNum=self.densityDensityNum.copy(),
+                    splusSminusNum=self.splusSminusNum.copy(), sminusSplusNum=self.sminusSplusNum.copy(),
+                    spairSpairNum=self.spairSpairNum.copy())
+
+
 # --------------------------------------------------------------------------
 # Real-materials molecule (phaseless)
 # --------------------------------------------------------------------------
