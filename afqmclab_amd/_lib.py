@@ -84,6 +84,15 @@ SIGNATURES = {
     "afqmc_cp_step": (C.c_int, [_vp, _vp, _dp, _ip]),
     "afqmc_cp_pop_control": (C.c_int, [_vp, _vp, _dp, _ip, _dp]),
     "afqmc_cp_measure": (C.c_int, [_vp, _vp]),
+    "afqmc_bp_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "afqmc_bp_destroy": (C.c_int, [_vp]),
+    "afqmc_bp_begin": (C.c_int, [_vp]),
+    "afqmc_bp_step": (C.c_int, [_vp, _vp]),
+    "afqmc_bp_pop_control": (C.c_int, [_vp, _vp, _dp]),
+    "afqmc_bp_measure": (C.c_int, [_vp, _vp]),
+    "afqmc_bp_get": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "afqmc_bp_reset": (C.c_int, [_vp]),
+    "afqmc_bp_get_step": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "afqmc_cp_energy": (C.c_int, [_vp, _dp]),
     "afqmc_cp_site_measure": (C.c_int, [_vp, _dp]),
     "afqmc_cp_reset_measure": (C.c_int, [_vp]),

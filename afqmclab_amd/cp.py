@@ -460,3 +460,168 @@ class AfqmcConstraintPath:
         H = sum(e[0] for e in self.energies)
         D = sum(e[1] for e in self.energies)
         return H / D
+
+
+# ------------------------------------------------------------------ back-propagation
+@dataclass
+class BPMethod(CPMethod):
+    """The afqmc_param keys of the back-propagation driver: the CP keys and backPropagationStep
+    (tutorials/afqmcConstraintPathBP/source/afqmcConstraintPathMethod.cpp:19-46, default 100)."""
+    backPropagationStep: int = 100
+
+    def analysis(self):
+        super().analysis()
+        if self.backPropagationStep < 0:
+            raise ValueError("backPropagationStep must be non-negative")
+
+
+BP_MATRICES = ("greenMatrixNum", "densityDensityNum", "splusSminusNum", "sminusSplusNum", "spairSpairNum")
+
+
+class BackPropagation:
+    """Device records and accumulators of the back-propagated estimator (afqmc_bp)."""
+
+    def __init__(self, prop: CPPropagator, nw, bp_steps, measure_skip, mgs):
+        h = C.c_void_p()
+        check(lib().afqmc_bp_create(prop.h, nw, bp_steps, measure_skip, mgs, C.byref(h)))
+        self.h = h
+        self.L = prop.model.L
+
+    def begin(self):
+        check(lib().afqmc_bp_begin(self.h))
+
+    def step(self, w):
+        check(lib().afqmc_bp_step(self.h, w.h))
+
+    def pop_control(self, w):
+        check(lib().afqmc_bp_pop_control(self.h, w.h, None))
+
+    def measure(self, w):
+        check(lib().afqmc_bp_measure(self.h, w.h))
+
+    def get(self):
+        """{den, HNum, KNum, VNum, RNum, greenMatrixNum, densityDensityNum, splusSminusNum,
+        sminusSplusNum, spairSpairNum} as HubbardSOCMeasureObserveSDSD holds them."""
+        L = self.L
+        sc = np.zeros(5, dtype=np.complex128)
+        g = np.zeros((2 * L, 2 * L), dtype=np.complex128, order="F")
+        dd = np.zeros((2 * L, 2 * L), dtype=np.complex128, order="F")
+        pm, mp, sp = (np.zeros((L, L), dtype=np.complex128, order="F") for _ in range(3))
+        check(lib().afqmc_bp_get(self.h, dptr(sc), dptr(g), dptr(dd), dptr(pm), dptr(mp), dptr(sp)))
+        out = dict(zip(["den", "HNum", "KNum", "VNum", "RNum"], sc))
+        out.update(zip(BP_MATRICES, (g, dd, pm, mp, sp)))
+        return out
+
+    def reset(self):
+        check(lib().afqmc_bp_reset(self.h))
+
+    def steps(self):
+        t, m = C.c_int64(), C.c_int64()
+        check(lib().afqmc_bp_get_step(self.h, C.byref(t), C.byref(m)))
+        return t.value, m.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().afqmc_bp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class AfqmcConstraintPathBP(AfqmcConstraintPath):
+    """The back-propagation driver (tutorials/afqmcConstraintPathBP/source/afqmcConstraintPath.cpp:115-180):
+    the constraint-path step loop with every step after thermalisation recorded on the device and a
+    back-propagated measurement where the reference's addMeasurement fires.  ``writes`` holds one
+    dict of accumulated numerators per write, as HubbardSOCMeasureObserveSDSD::write puts them on
+    disk; ``betas`` the matching beta.dat values."""
+
+    def __init__(self, ctx, model, method: BPMethod, phiT, nup=None, spin_blocks=None, precision=0, uniforms=None,
+                 seed=None):
+        super().__init__(ctx, model, method, phiT, nup=nup, spin_blocks=spin_blocks, precision=precision,
+                         uniforms=uniforms, seed=seed)
+        self.bp = BackPropagation(self.prop, method.walkerSizePerThread, method.backPropagationStep,
+                                  method.measureSkipStep, method.mgsStep)
+        self.recording = False
+        self.isBP = False
+        self.BPIndex = 0
+        self.writes = []
+        self.betas = []
+
+    def projectOneStep(self, idx):
+        if self.recording:
+            self.bp.step(self.walker)
+        else:
+            self.prop.step(self.walker)
+        self.steps_done += 1
+        self.walker_steps += self.walker.nw
+        idx[0] += 1
+        if idx[0] == self.method.mgsStep:
+            self.prop.stabilize(self.walker)
+            idx[0] = 0
+        idx[1] += 1
+        if idx[1] == self.method.popControlStep:
+            if self.recording:
+                self.bp.pop_control(self.walker)
+            else:
+                self.prop.pop_control(self.walker)
+            idx[1] = 0
+        if self.isBP:
+            self.BPIndex += 1
+
+    def prepareBackPropagation(self):
+        """afqmcConstraintPathBackPropagation.cpp:11-20; the device keeps recording from the first call."""
+        if not self.recording:
+            self.bp.begin()
+            self.recording = True
+        self.isBP = True
+        self.BPIndex = 0
+
+    def addMeasurement(self):
+        """afqmcConstraintPathMeasure.cpp:41-72 with the window bookkeeping of
+        prepareBackPropagationMeasAndNextBackup (afqmcConstraintPathBackPropagation.cpp:59-76)."""
+        if not self.isBP:
+            return
+        m = self.method
+        self.bp.measure(self.walker)
+        if m.backPropagationStep >= m.measureSkipStep:
+            self.BPIndex = m.backPropagationStep - m.measureSkipStep
+        else:
+            self.isBP = False
+
+    def measureWithProjection(self, on_block=None):
+        m = self.method
+        nbp, skip = m.backPropagationStep, m.measureSkipStep
+        self.prop.onebody(self.walker, 1)
+        idx = [0, 0]
+        for i in range(m.thermalSize):
+            if i < m.ETAdjustMaxSize and i % m.ETAdjustStep == 0:
+                self.prop.measure(self.walker)
+                self.prop.adjust_ET()
+            self.projectOneStep(idx)
+        self.prepareBackPropagation()
+        if nbp >= skip:
+            for _ in range(nbp):
+                self.projectOneStep(idx)
+        for i in range(m.writeNumber):
+            for j in range(m.measureNumberPerWrite):
+                for k in range(skip):
+                    if self.BPIndex == nbp:
+                        self.addMeasurement()
+                    self.projectOneStep(idx)
+                if nbp < skip:
+                    self.prepareBackPropagation()
+            self.betas.append((m.thermalSize + (i + 0.5) * m.measureNumberPerWrite * skip - 0.5) * m.dt)
+            w = self.bp.get()
+            self.writes.append(w)
+            self.bp.reset()
+            if on_block:
+                on_block(i, w)
+
+    def energy(self):
+        H = sum(w["HNum"] for w in self.writes)
+        D = sum(w["den"] for w in self.writes)
+        return H / D

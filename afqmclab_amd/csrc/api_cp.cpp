@@ -101,7 +101,7 @@ HsSampleArgs hs_args(afqmc_cp cp, afqmc_walkers w) {
 }
 
 template <typename T>
-int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out) {
+int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out, int* aux_dev = nullptr) {
   hipStream_t s = cp->ctx->stream;
   int rc;
   REQUIRE(!capturing() || (!force_out && !aux_out), "a captured step cannot return records");
@@ -126,6 +126,7 @@ int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out) {
     h.force_out = cp->force_rec.p;
     h.aux_out = cp->aux_rec.p;
   }
+  if (aux_dev) h.aux_out = aux_dev;
   {
     RegionTimer t(cp->ctx, REG_HS_SAMPLE);
     if ((rc = launch_hs_sample(h, s))) return rc;
@@ -270,9 +271,30 @@ int measure_impl(afqmc_cp cp, afqmc_walkers w) {
   return launch_energy(e, s);
 }
 
+// den[w] = exp(logO) of phiT and exp(dt/2 K) W_w, the weight of addMixedMeasurement
+// (afqmcConstraintPathMeasure.cpp:14-26), without accumulating the energy
+template <typename T>
+int mixed_weight_impl(afqmc_cp cp, afqmc_walkers w) {
+  hipStream_t s = cp->ctx->stream;
+  int rc;
+  if ((rc = propagate_gemm<T>(cp, w, 2, w->wf, w->tmp, false))) return rc;
+  if ((rc = launch_logw_shift(w->nw, w->logw.p, w->logw_tmp.p, cp->hop_logw[2], s))) return rc;
+  if ((rc = overlap_gemm<T>(cp, w, w->tmp))) return rc;
+  if ((rc = gj_blocks<T>(cp, w, false, w->alive.p))) return rc;
+  LogOArgs l = logo_args<T>(cp, w, LOGO_MEASURE);
+  l.logw = w->logw_tmp.p;
+  return launch_logo(l, s);
+}
+
 }  // namespace
 
 #define DISPATCH(cp, fn, ...) ((cp)->fp32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+
+int cpi::step_record(afqmc_cp cp, afqmc_walkers w, int* aux_dev) {
+  return DISPATCH(cp, step_impl, cp, w, nullptr, nullptr, aux_dev);
+}
+int cpi::mixed_weight(afqmc_cp cp, afqmc_walkers w) { return DISPATCH(cp, mixed_weight_impl, cp, w); }
+int cpi::stabilize(afqmc_cp cp, afqmc_walkers w) { return stabilize_impl(cp, w); }
 
 // ------------------------------------------------------------------ C entry points
 namespace {
@@ -399,6 +421,8 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
   // trial, propagators, (phiT^H K)^T
   const int L2 = 2 * L;
   std::vector<cd> K((const cd*)d->K, (const cd*)d->K + (size_t)L2 * L2);
+  cp->h_K = K;
+  cp->h_U = U; cp->h_mu = mu; cp->h_hx = hx; cp->h_hy = hy; cp->h_hz = hz;
   std::vector<cd> phi((const cd*)d->phiT, (const cd*)d->phiT + (size_t)L2 * d->N);
   std::vector<cd> wdkT((size_t)L2 * d->N);  // (phiT^H K)^T = K^T conj(phiT): 2L x N
   for (int n = 0; n < d->N; ++n)

@@ -38,6 +38,8 @@ struct afqmc_cp_s {
   int ntile[2] = {1, 1};
   int tile[3] = {GEMM_TILE_64x64, GEMM_TILE_64x64, GEMM_TILE_64x64};  // propagate, overlap, green
   std::vector<cd> h_gamma, h_cdiag;
+  std::vector<cd> h_K;                                 // 2L x 2L, for energies from Green matrices
+  std::vector<double> h_U, h_mu, h_hx, h_hy, h_hz;
   size_t esz() const { return fp32 ? sizeof(zf) : sizeof(zd); }
 };
 
@@ -163,5 +165,12 @@ int propagate_gemm(afqmc_cp cp, afqmc_walkers w, int which, DevBuf<char>* in, De
   }
   return 0;
 }
+
+// the walker step with each alive walker's aux fields written to aux_dev ([nw][L], device)
+int step_record(afqmc_cp cp, afqmc_walkers w, int* aux_dev);
+// cp->den[w] = the mixed-estimator weight of walker w (all walkers; dead ones are masked by the user)
+int mixed_weight(afqmc_cp cp, afqmc_walkers w);
+// SD::stabilize of every alive walker, enqueued without synchronising
+int stabilize(afqmc_cp cp, afqmc_walkers w);
 
 }  // namespace cpi

@@ -159,6 +159,37 @@ __device__ __forceinline__ zd csin_c(zd a) {                                    
   return cmk<double>(s.im, -s.re);
 }
 
+// The 2x2 site matrix [d00 d01; d10 d11] acting on (up_i, dn_i) of the sample with gamma_i x_i = gx
+// (NiupNidn.cpp:269-322): the field exponential times the constant one-body factor constDiag.
+__device__ __forceinline__ void hs_site_matrix(int decomp, zd gx, const zd* cdiag, int L, int i, zd& d00, zd& d01,
+                                               zd& d10, zd& d11) {
+  const zd c00 = cdiag[i], c01 = cdiag[L + i], c10 = cdiag[2 * L + i], c11 = cdiag[3 * L + i];
+  switch (decomp) {
+    case DECOMP_DENSITY_CHARGE: {
+      zd e = cexp_c(gx);
+      d00 = cmul(e, c00); d10 = cmul(e, c10); d01 = cmul(e, c01); d11 = cmul(e, c11);
+      break;
+    }
+    case DECOMP_DENSITY_SPIN: {
+      zd e = cexp_c(gx), einv = cinv(e);
+      d00 = cmul(e, c00); d10 = cmul(einv, c10); d01 = cmul(e, c01); d11 = cmul(einv, c11);
+      break;
+    }
+    case DECOMP_HOP_CHARGE: {
+      zd ch = ccosh_c(gx), sn = csinh_c(gx);
+      d00 = cadd(cmul(ch, c00), cmul(sn, c10)); d10 = cadd(cmul(sn, c00), cmul(ch, c10));
+      d01 = cadd(cmul(ch, c01), cmul(sn, c11)); d11 = cadd(cmul(sn, c01), cmul(ch, c11));
+      break;
+    }
+    default: {
+      zd co = ccos_c(gx), si = csin_c(gx);
+      d00 = cadd(cmul(co, c00), cmul(si, c10)); d10 = csub(cmul(co, c10), cmul(si, c00));
+      d01 = cadd(cmul(co, c01), cmul(si, c11)); d11 = csub(cmul(co, c11), cmul(si, c01));
+      break;
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
   const int w = blockIdx.x;
@@ -195,32 +226,8 @@ __global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
     lp += x * f - log(ep + em);
     const zd gx = cscale(a.gamma[i], (double)x);
     xg = cadd(xg, gx);
-    const zd c00 = a.cdiag[i], c01 = a.cdiag[L + i], c10 = a.cdiag[2 * L + i], c11 = a.cdiag[3 * L + i];
     zd d00, d01, d10, d11;
-    switch (a.decomp) {
-      case DECOMP_DENSITY_CHARGE: {
-        zd e = cexp_c(gx);
-        d00 = cmul(e, c00); d10 = cmul(e, c10); d01 = cmul(e, c01); d11 = cmul(e, c11);
-        break;
-      }
-      case DECOMP_DENSITY_SPIN: {
-        zd e = cexp_c(gx), einv = cinv(e);
-        d00 = cmul(e, c00); d10 = cmul(einv, c10); d01 = cmul(e, c01); d11 = cmul(einv, c11);
-        break;
-      }
-      case DECOMP_HOP_CHARGE: {
-        zd ch = ccosh_c(gx), sn = csinh_c(gx);
-        d00 = cadd(cmul(ch, c00), cmul(sn, c10)); d10 = cadd(cmul(sn, c00), cmul(ch, c10));
-        d01 = cadd(cmul(ch, c01), cmul(sn, c11)); d11 = cadd(cmul(sn, c01), cmul(ch, c11));
-        break;
-      }
-      default: {
-        zd co = ccos_c(gx), si = csin_c(gx);
-        d00 = cadd(cmul(co, c00), cmul(si, c10)); d10 = csub(cmul(co, c10), cmul(si, c00));
-        d01 = cadd(cmul(co, c01), cmul(si, c11)); d11 = csub(cmul(co, c11), cmul(si, c01));
-        break;
-      }
-    }
+    hs_site_matrix(a.decomp, gx, a.cdiag, L, i, d00, d01, d10, d11);
     if (a.diag_only) {
       if (a.dense) {
         ct* s0 = reinterpret_cast<ct*>(a.scale[0]) + (long long)w * a.Mb[0];
@@ -483,6 +490,203 @@ __global__ void logw_add_kernel(int nw, zd* logw, const int* alive, const zd* ad
   logw[w].re += d;
 }
 
+// ------------------------------------------------------------------ back-propagation
+template <typename T>
+__global__ __launch_bounds__(256) void bp_field_kernel(BpFieldArgs a) {
+  const int w = blockIdx.x;
+  __shared__ int anc_sh;
+  if (threadIdx.x == 0) {
+    int p = a.anc[w];
+    if (a.table) p = a.table[p];
+    if (p < 0 || p >= a.nw) p = w;  // a comb over zero total weight leaves no valid parent
+    a.anc[w] = p;
+    anc_sh = p;
+  }
+  __syncthreads();
+  const int L = a.L;
+  const int* x = a.aux + (long long)anc_sh * L;
+  typedef cplx<T> ct;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    zd d00, d01, d10, d11;
+    hs_site_matrix(a.decomp, cscale(a.gamma[i], (double)x[i]), a.cdiag, L, i, d00, d01, d10, d11);
+    // applyToLeft multiplies by D^H: [conj d00, conj d10; conj d01, conj d11]
+    const zd e00 = cconj(d00), e01 = cconj(d10), e10 = cconj(d01), e11 = cconj(d11);
+    if (a.diag_only) {
+      if (a.dense) {
+        ct* s0 = reinterpret_cast<ct*>(a.scale[0]) + (long long)w * a.Mb[0];
+        s0[i] = from_zd<T>(e00);
+        s0[i + L] = from_zd<T>(e11);
+      } else {
+        reinterpret_cast<ct*>(a.scale[0])[(long long)w * a.Mb[0] + i] = from_zd<T>(e00);
+        reinterpret_cast<ct*>(a.scale[1])[(long long)w * a.Mb[1] + i] = from_zd<T>(e11);
+      }
+    } else {
+      ct* d = reinterpret_cast<ct*>(a.d2x2) + (long long)w * 4 * L;
+      d[i] = from_zd<T>(e00);
+      d[L + i] = from_zd<T>(e01);
+      d[2 * L + i] = from_zd<T>(e10);
+      d[3 * L + i] = from_zd<T>(e11);
+    }
+  }
+}
+
+__global__ void iota_kernel(int n, int* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i;
+}
+
+template <typename E>
+__global__ void gather_walkers_kernel(long long n, const int* idx, const E* __restrict__ src, E* __restrict__ dst) {
+  const int w = blockIdx.y;
+  const E* s = src + (idx ? (long long)idx[w] * n : 0);
+  E* d = dst + (long long)w * n;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    d[k] = s[k];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void conj_transpose_kernel(int M, int N, long long sstride, long long dstride,
+                                                             const int* alive, const cplx<T>* __restrict__ src,
+                                                             cplx<T>* __restrict__ dst) {
+  const int w = blockIdx.z;
+  if (alive && !alive[w]) return;
+  __shared__ cplx<T> t[32][33];
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const cplx<T>* sp = src + (long long)w * sstride;
+  for (int r = ty; r < 32; r += 8) {
+    const int m = m0 + tx, n = n0 + r;
+    if (m < M && n < N) t[r][tx] = sp[m + (long long)n * M];
+  }
+  __syncthreads();
+  cplx<T>* dp = dst + (long long)w * dstride;
+  for (int r = ty; r < 32; r += 8) {
+    const int n = n0 + tx, m = m0 + r;
+    if (m < M && n < N) dp[n + (long long)m * N] = cconj(t[tx][r]);
+  }
+}
+
+// G(r, c) of one walker: dense G^T with leading dimension ld0, or the up-up / down-down blocks
+template <typename T>
+__device__ __forceinline__ zd wick_g(const cplx<T>* g0, const cplx<T>* g1, int dense, int ld0, int L, int r, int c) {
+  if (dense) return to_zd(g0[c + (long long)r * ld0]);
+  if (r < L && c < L) return to_zd(g0[c + (long long)r * L]);
+  if (r >= L && c >= L) return to_zd(g1[(c - L) + (long long)(r - L) * L]);
+  return cmk<double>(0, 0);
+}
+
+__device__ __forceinline__ void cfma_acc(zd& acc, zd w, zd v) { acc = cadd(acc, cmul(w, v)); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void wick_kernel(WickArgs a) {
+  const int L = a.L;
+  const int ntI = (L + 15) / 16;
+  const int I0 = (blockIdx.x % ntI) * 16, J0 = (blockIdx.x / ntI) * 16;
+  const int tj = threadIdx.x & 15, ti = threadIdx.x >> 4;
+  const int i = I0 + ti, j = J0 + tj;
+  const bool act = i < L && j < L;
+  // tr[q][r][c] = G(J0 + r + sj L, I0 + c + si L) with q = 2 sj + si
+  __shared__ zd tr[4][16][17];
+  // dg[side][q][k]: side 0 the i rows, 1 the j rows; q: G(k,k), G(k+L,k+L), G(k,k+L), G(k+L,k)
+  __shared__ zd dg[2][4][16];
+  zd acc[WICK_Q];
+#pragma unroll
+  for (int q = 0; q < WICK_Q; ++q) acc[q] = cmk<double>(0, 0);
+  const zd zero = cmk<double>(0, 0);
+  typedef cplx<T> ct;
+  for (int w = a.c0 + (int)blockIdx.y; w < a.c0 + a.cn; w += a.slices) {
+    if (!a.alive[w]) continue;
+    const zd wt = a.den[w];
+    const ct* g0 = reinterpret_cast<const ct*>(a.gt[0]) + (long long)(w - a.c0) * a.gstride[0];
+    const ct* g1 = a.dense ? nullptr : reinterpret_cast<const ct*>(a.gt[1]) + (long long)(w - a.c0) * a.gstride[1];
+    __syncthreads();
+    {
+      const int jj = J0 + ti, ii = I0 + tj;
+      const bool ok = jj < L && ii < L;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int sj = q >> 1, si = q & 1;
+        tr[q][ti][tj] = ok ? wick_g<T>(g0, g1, a.dense, a.ld[0], L, jj + sj * L, ii + si * L) : zero;
+      }
+    }
+    if (threadIdx.x < 128) {
+      const int side = threadIdx.x >> 6, q = (threadIdx.x >> 4) & 3, k = threadIdx.x & 15;
+      const int kk = (side ? J0 : I0) + k;
+      const int ro = (q == 1 || q == 3) ? L : 0, co = (q == 1 || q == 2) ? L : 0;
+      dg[side][q][k] = kk < L ? wick_g<T>(g0, g1, a.dense, a.ld[0], L, kk + ro, kk + co) : zero;
+    }
+    zd gUU = zero, gUD = zero, gDU = zero, gDD = zero;
+    if (act) {
+      gUU = wick_g<T>(g0, g1, a.dense, a.ld[0], L, i, j);
+      gDD = wick_g<T>(g0, g1, a.dense, a.ld[0], L, i + L, j + L);
+      if (a.dense) {
+        gUD = wick_g<T>(g0, g1, 1, a.ld[0], L, i, j + L);
+        gDU = wick_g<T>(g0, g1, 1, a.ld[0], L, i + L, j);
+      }
+    }
+    __syncthreads();
+    if (!act) continue;
+    const zd tUU = tr[0][tj][ti], tUD = tr[1][tj][ti], tDU = tr[2][tj][ti], tDD = tr[3][tj][ti];
+    const zd diU = dg[0][0][ti], diD = dg[0][1][ti], diUD = dg[0][2][ti], diDU = dg[0][3][ti];
+    const zd djU = dg[1][0][tj], djD = dg[1][1][tj], djUD = dg[1][2][tj], djDU = dg[1][3][tj];
+    cfma_acc(acc[0], wt, gUU);
+    cfma_acc(acc[1], wt, gUD);
+    cfma_acc(acc[2], wt, gDU);
+    cfma_acc(acc[3], wt, gDD);
+    cfma_acc(acc[5], wt, csub(cmul(diU, djD), cmul(gUD, tDU)));
+    cfma_acc(acc[6], wt, csub(cmul(diD, djU), cmul(gDU, tUD)));
+    cfma_acc(acc[10], wt, csub(cmul(gUU, gDD), cmul(gUD, gDU)));
+    if (i == j) {
+      const zd det = csub(cmul(diU, diD), cmul(diUD, diDU));
+      cfma_acc(acc[4], wt, diU);
+      cfma_acc(acc[7], wt, diD);
+      cfma_acc(acc[8], wt, csub(diU, det));
+      cfma_acc(acc[9], wt, csub(diD, det));
+    } else {
+      cfma_acc(acc[4], wt, csub(cmul(diU, djU), cmul(gUU, tUU)));
+      cfma_acc(acc[7], wt, csub(cmul(diD, djD), cmul(gDD, tDD)));
+      cfma_acc(acc[8], wt, csub(cmul(diUD, djDU), cmul(gUU, tDD)));
+      cfma_acc(acc[9], wt, csub(cmul(diDU, djUD), cmul(gDD, tUU)));
+    }
+  }
+  if (!act) return;
+  const long long LL = (long long)L * L, pr = i + (long long)j * L;
+  zd* out = a.part + (long long)blockIdx.y * WICK_Q * LL + pr;
+#pragma unroll
+  for (int q = 0; q < WICK_Q; ++q) out[q * LL] = a.first ? acc[q] : cadd(out[q * LL], acc[q]);
+}
+
+__global__ void wick_reduce_kernel(int L, int slices, const zd* part, zd* green, zd* dd, zd* pm, zd* mp, zd* sp) {
+  const long long LL = (long long)L * L;
+  const long long pr = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pr >= LL) return;
+  const int i = (int)(pr % L), j = (int)(pr / L);
+  zd v[WICK_Q];
+  for (int q = 0; q < WICK_Q; ++q) {
+    zd t = cmk<double>(0, 0);
+    for (int y = 0; y < slices; ++y) t = cadd(t, part[((long long)y * WICK_Q + q) * LL + pr]);
+    v[q] = t;
+  }
+  const long long L2 = 2 * (long long)L;
+  const long long at[4] = {i + j * L2, i + (j + L) * L2, (i + L) + j * L2, (i + L) + (j + L) * L2};
+  for (int s = 0; s < 4; ++s) {
+    green[at[s]] = cadd(green[at[s]], v[s]);
+    dd[at[s]] = cadd(dd[at[s]], v[4 + s]);
+  }
+  pm[pr] = cadd(pm[pr], v[8]);
+  mp[pr] = cadd(mp[pr], v[9]);
+  sp[pr] = cadd(sp[pr], v[10]);
+}
+
+__global__ void den_sum_kernel(int nw, const zd* den, const int* alive, zd* acc) {
+  __shared__ double sh[16];
+  zd t = cmk<double>(0, 0);
+  for (int w = threadIdx.x; w < nw; w += blockDim.x)
+    if (alive[w]) t = cadd(t, den[w]);
+  t = block_sum_z(t, sh);
+  if (threadIdx.x == 0) acc[0] = cadd(acc[0], t);
+}
+
 }  // namespace
 
 int launch_logo(const LogOArgs& a, hipStream_t s) {
@@ -574,6 +778,71 @@ int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, con
                          double mult0, double mult1, hipStream_t s) {
   hipLaunchKernelGGL(logw_add_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, nw, logw, alive, add0, add1, mult0,
                      mult1);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_bp_field(const BpFieldArgs& a, hipStream_t s) {
+  if (a.nw <= 0) return 0;
+  if (a.fp32) hipLaunchKernelGGL(bp_field_kernel<float>, dim3(a.nw), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(bp_field_kernel<double>, dim3(a.nw), dim3(256), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_iota(int n, int* out, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, out);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_gather_walkers(int nw, long long n, size_t esz, const int* idx, const void* src, void* dst, hipStream_t s) {
+  if (nw <= 0 || n <= 0) return 0;
+  dim3 grid((unsigned)std::min<long long>((n + 255) / 256, 256), nw);
+  if (esz == 16)
+    hipLaunchKernelGGL(gather_walkers_kernel<uint4>, grid, dim3(256), 0, s, n, idx, (const uint4*)src, (uint4*)dst);
+  else
+    hipLaunchKernelGGL(gather_walkers_kernel<uint2>, grid, dim3(256), 0, s, n, idx, (const uint2*)src, (uint2*)dst);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+int launch_conj_transpose(int nw, int M, int N, long long sstride, long long dstride, const int* alive,
+                          const void* src, void* dst, hipStream_t s) {
+  if (nw <= 0 || M <= 0 || N <= 0) return 0;
+  dim3 grid((M + 31) / 32, (N + 31) / 32, nw);
+  hipLaunchKernelGGL(conj_transpose_kernel<T>, grid, dim3(256), 0, s, M, N, sstride, dstride, alive,
+                     reinterpret_cast<const cplx<T>*>(src), reinterpret_cast<cplx<T>*>(dst));
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+template int launch_conj_transpose<double>(int, int, int, long long, long long, const int*, const void*, void*,
+                                           hipStream_t);
+template int launch_conj_transpose<float>(int, int, int, long long, long long, const int*, const void*, void*,
+                                          hipStream_t);
+
+int launch_wick(const WickArgs& a, hipStream_t s) {
+  const int nt = (a.L + 15) / 16;
+  dim3 grid(nt * nt, a.slices);
+  if (a.fp32) hipLaunchKernelGGL(wick_kernel<float>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(wick_kernel<double>, grid, dim3(256), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_wick_reduce(int L, int slices, const zd* part, zd* acc_green, zd* acc_dd, zd* acc_pm, zd* acc_mp,
+                       zd* acc_sp, hipStream_t s) {
+  const long long LL = (long long)L * L;
+  hipLaunchKernelGGL(wick_reduce_kernel, dim3((unsigned)((LL + 255) / 256)), dim3(256), 0, s, L, slices, part,
+                     acc_green, acc_dd, acc_pm, acc_mp, acc_sp);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_den_sum(int nw, const zd* den, const int* alive, zd* acc, hipStream_t s) {
+  hipLaunchKernelGGL(den_sum_kernel, dim3(1), dim3(256), 0, s, nw, den, alive, acc);
   HIP_LAUNCH_CHECK();
   return 0;
 }

@@ -110,3 +110,46 @@ int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol,
                        hipStream_t s);
 int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int use_imag_sub,
                          double mult0, double mult1, hipStream_t s);
+
+// ------------------------------------------------------------------ back-propagation
+// One backward step of the per-walker bras: anc[w] = table[anc[w]] when the step had a comb, then the
+// conjugate transposes D^H of the site matrices of the ancestor's recorded fields, in the layout the
+// walker step writes (scale per block for diagonal samples, d2x2 otherwise).
+struct BpFieldArgs {
+  int nw, L, decomp, dense, diag_only;
+  const int* aux;    // [nw][L] fields of this step, by slot
+  const int* table;  // comb table of this step or nullptr
+  int* anc;          // [nw] ancestor slots, updated in place
+  const zd* gamma; const zd* cdiag;
+  void* scale[2]; int Mb[2];
+  void* d2x2;
+  int fp32;
+};
+int launch_bp_field(const BpFieldArgs& a, hipStream_t s);
+int launch_iota(int n, int* out, hipStream_t s);
+// dst[w] = src[idx[w]] for per-walker blocks of n complex elements; idx == nullptr: dst[w] = src[0]
+int launch_gather_walkers(int nw, long long n, size_t esz, const int* idx, const void* src, void* dst, hipStream_t s);
+// dst_w = src_w^H: src M x N (ld M), dst N x M (ld N), walkers with alive[w] != 0
+template <typename T>
+int launch_conj_transpose(int nw, int M, int N, long long sstride, long long dstride, const int* alive,
+                          const void* src, void* dst, hipStream_t s);
+
+// Accumulation of the observables of HubbardSOCMeasureObserveSDSD from per-walker Green matrices:
+// G(r, c) = gt[c + r * ld] (a walker's G^T, column-major), spin blocks (dense = 0) hold the up-up and
+// down-down L x L blocks and zero spin-flip blocks.  Each thread owns a site pair (i, j) and sums, over
+// the walkers of the chunk with weight den[w], 11 complex quantities: G at the four spin pairs,
+// <n_a n_b> at the four spin pairs, <S+_i S-_j>, <S-_i S+_j>, <D+_i D_j>.
+constexpr int WICK_Q = 11;
+struct WickArgs {
+  int L, dense, c0, cn, slices, first;
+  const void* gt[2]; long long gstride[2]; int ld[2];
+  const zd* den; const int* alive;
+  zd* part;  // [slices][WICK_Q][L * L] (pair index i + j L)
+  int fp32;
+};
+int launch_wick(const WickArgs& a, hipStream_t s);
+// acc_green (2L x 2L), acc_dd (2L x 2L), acc_pm, acc_mp, acc_sp (L x L) += sum over slices of part
+int launch_wick_reduce(int L, int slices, const zd* part, zd* acc_green, zd* acc_dd, zd* acc_pm, zd* acc_mp,
+                       zd* acc_sp, hipStream_t s);
+// acc[0] += sum of den[w] over alive walkers
+int launch_den_sum(int nw, const zd* den, const int* alive, zd* acc, hipStream_t s);

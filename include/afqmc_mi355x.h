@@ -189,6 +189,37 @@ int afqmc_cp_fixed_field_energy(afqmc_cp cp, afqmc_walkers w, int nsteps, int mg
 int afqmc_cp_fixed_field_adjoint(afqmc_cp cp, afqmc_walkers w, int nsteps, int mgs, int ckpt, uint64_t field_seed,
                                  double* energy, double* bbar, double* mbar, double* ubar);
 
+/* ------------------------------------------------------------------ back-propagation */
+/* The back-propagated estimator of tutorials/afqmcConstraintPathBP: the aux fields of every step
+ * (twoBodyAuxBackup, afqmcConstraintPathProjection.cpp:73) and the comb tables
+ * (tablePerThreadBackup, :163-171) are kept on the device for the last bp_steps
+ * (backPropagationStep) steps, the walkers are copied every measure_skip (measureSkipStep) steps,
+ * and a measurement propagates phiT to the left through each walker's fields along its comb
+ * ancestry, normalising every mgs (mgsStep) steps (afqmcConstraintPathMeasure.cpp:41-72;
+ * afqmcConstraintPathBackPropagation.cpp:11-97).  The observables are those of
+ * HubbardSOCMeasureObserveSDSD (HubbardSOCMeasureObserveSDSD.cpp:34-183), weighted by each walker's
+ * mixed overlap.  The propagators must be Hermitian (exp(-alpha K) of a Hermitian K, as
+ * HubbardSOC::returnExpMinusAlphaK builds them), since applyToLeft multiplies by their adjoints. */
+typedef struct afqmc_bp_s* afqmc_bp;
+int afqmc_bp_create(afqmc_cp cp, int nw, int bp_steps, int measure_skip, int mgs, afqmc_bp* out);
+int afqmc_bp_destroy(afqmc_bp bp);
+/* prepareBackPropagation (afqmcConstraintPathBackPropagation.cpp:11-20): step 0 of the recording */
+int afqmc_bp_begin(afqmc_bp bp);
+/* afqmc_cp_step with the aux fields recorded (and the walkers copied when a window starts) */
+int afqmc_bp_step(afqmc_bp bp, afqmc_walkers w);
+/* afqmc_cp_pop_control after the last afqmc_bp_step, its table recorded; stats may be NULL */
+int afqmc_bp_pop_control(afqmc_bp bp, afqmc_walkers w, double* stats);
+/* addMeasurement at step k * measure_skip + bp_steps of the recording, accumulating on the device */
+int afqmc_bp_measure(afqmc_bp bp, afqmc_walkers w);
+/* the accumulated numerators (complex, column-major), any pointer may be NULL:
+ * scalars: den, HNum, KNum, VNum, RNum (5 complex); green: greenMatrixNum (2L x 2L);
+ * dd: densityDensityNum (2L x 2L); pm, mp, sp: splusSminusNum, sminusSplusNum, spairSpairNum (L x L) */
+int afqmc_bp_get(afqmc_bp bp, double* scalars, double* green, double* dd, double* pm, double* mp, double* sp);
+/* observeMeasure.reSet after a write */
+int afqmc_bp_reset(afqmc_bp bp);
+/* steps recorded since afqmc_bp_begin and measurements made */
+int afqmc_bp_get_step(afqmc_bp bp, int64_t* steps, int64_t* measurements);
+
 /* ------------------------------------------------------------------ phaseless (real materials) */
 /* Model, trial and method of the phaseless driver: RealMaterialMolecule (realMaterialMolecule.h:16-40,
  * t, choleskyVecs, choleskyBg), the SD2s trial, the half-rotated vectors of

@@ -1,0 +1,135 @@
+"""Back-propagation on the GPU against the oracle's back-propagation driver, through the C-ABI.
+
+The oracle moves recorded fields and backups with every comb table and propagates backups again, as
+the reference does; the device keeps a ring of fields and tables, copies the walkers at window
+starts and follows each walker's ancestry.  Both run the same step loop on the same uniform stream,
+so the numerators of every write (den, HNum, KNum, VNum, RNum and the five observable matrices)
+agree to 1e-9 relative in FP64.  A U = 0 run at a larger size is checked against the closed form."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from afqmclab_amd.cp import AfqmcConstraintPathBP, BPMethod as DevBP  # noqa: E402
+from afqmclab_amd.model import HubbardModel, free_electron_trial, spin_block_trial  # noqa: E402
+from oracle.blocks import SD  # noqa: E402
+from oracle.drivers import AfqmcConstraintPathBP as OracleBP, BPMethod as OracleBPMethod  # noqa: E402
+from oracle.models import HubbardSOC  # noqa: E402
+from oracle.rng import UniformStream  # noqa: E402
+from tests.gpu_util import ctx  # noqa: E402
+
+KEYS = ("den", "HNum", "KNum", "VNum", "RNum", "greenMatrixNum", "densityDensityNum", "splusSminusNum",
+        "sminusSplusNum", "spairSpairNum")
+
+
+def compare_writes(dev, ref, rtol):
+    assert len(dev) == len(ref)
+    for a, b in zip(dev, ref):
+        for k in KEYS:
+            x, y = np.asarray(a[k]), np.asarray(b[k])
+            scale = max(np.max(np.abs(y)), abs(b["den"]) * 1e-3)
+            err = np.max(np.abs(x - y))
+            assert err <= rtol * scale, (k, err, scale)
+
+
+def run_pair(model, phi, m, kw, seed, precision=0):
+    dev = AfqmcConstraintPathBP(ctx(), model, DevBP(**m), phi, precision=precision, **kw)
+    dev.run()
+    om = HubbardSOC(model.L, model.N, model.K, mu=model.mu, hx=model.hx, hy=model.hy, hz=model.hz, U=model.U)
+    ref = OracleBP(OracleBPMethod(**m), om, UniformStream(seed), phiT=SD(wf=phi))
+    ref.run()
+    assert dev.prop.counter() == ref.rng.counter
+    return dev, ref
+
+
+def mixing_model(seed=31, U=6.0):
+    rng = np.random.default_rng(seed)
+    base = HubbardModel.square(2, 2, 3, U)
+    L = base.L
+    return HubbardModel(L=L, N=3, K=base.K, U=np.full(L, U), mu=rng.uniform(-0.3, 0.3, L),
+                        hx=rng.uniform(-0.5, 0.5, L), hy=rng.uniform(-0.5, 0.5, L), hz=rng.uniform(-0.5, 0.5, L))
+
+
+@pytest.mark.parametrize("nbp,skip", [(6, 4), (4, 6), (5, 5), (0, 3)])
+def test_bp_spin_mixing_matches_oracle(nbp, skip):
+    """Dense 2L x N walkers with hx, hy != 0 (2x2 site matrices), densityCharge (complex fields, so
+    walkers die and the comb copies), comb every 2 steps."""
+    model = mixing_model()
+    phi = free_electron_trial(model)
+    seed = 123
+    m = dict(dt=0.08, thermalSize=6, writeNumber=2, measureNumberPerWrite=2, measureSkipStep=skip,
+             walkerSizePerThread=10, decompType="densityCharge", forceType="dynamicForce", mgsStep=3,
+             popControlStep=2, ET=-3.0, ETAdjustStep=3, ETAdjustMaxSize=6, backPropagationStep=nbp, seed=seed)
+    dev, ref = run_pair(model, phi, m, dict(spin_blocks=False), seed)
+    compare_writes(dev.writes, ref.writes, 1e-9)
+    assert dev.betas == pytest.approx(ref.betas)
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_bp_hubbard_matches_oracle(mode):
+    """4x4 Hubbard, densitySpin, diagonal site matrices: dense walkers and spin blocks."""
+    nx, ny, nup, ndn, U = 4, 4, 5, 5, 4.0
+    model = HubbardModel.square(nx, ny, nup + ndn, U)
+    phi = free_electron_trial(model) if mode == "dense" else spin_block_trial(model, nup, ndn)
+    kw = dict(spin_blocks=False) if mode == "dense" else dict(nup=nup, spin_blocks=True)
+    seed = 2024
+    m = dict(dt=0.05, thermalSize=10, writeNumber=2, measureNumberPerWrite=2, measureSkipStep=4,
+             walkerSizePerThread=12, decompType="densitySpin", mgsStep=4, popControlStep=5,
+             ETAdjustStep=5, ETAdjustMaxSize=10, backPropagationStep=10, seed=seed)
+    dev, ref = run_pair(model, phi, m, kw, seed)
+    compare_writes(dev.writes, ref.writes, 1e-9)
+
+
+def test_bp_mixed_precision_close_to_oracle():
+    """FP32 walkers and GEMMs: the same run to the mixed-precision bar (energies 1e-4 relative)."""
+    nx, ny, nup, ndn, U = 4, 4, 5, 5, 4.0
+    model = HubbardModel.square(nx, ny, nup + ndn, U)
+    phi = spin_block_trial(model, nup, ndn)
+    seed = 7
+    m = dict(dt=0.05, thermalSize=10, writeNumber=1, measureNumberPerWrite=2, measureSkipStep=4,
+             walkerSizePerThread=12, decompType="densitySpin", mgsStep=4, popControlStep=5,
+             ETAdjustStep=5, ETAdjustMaxSize=10, backPropagationStep=8, seed=seed)
+    dev, ref = run_pair(model, phi, m, dict(nup=nup, spin_blocks=True), seed, precision=1)
+    e_dev = dev.writes[0]["HNum"] / dev.writes[0]["den"]
+    e_ref = ref.writes[0]["HNum"] / ref.writes[0]["den"]
+    assert abs(e_dev - e_ref) < 1e-4 * abs(e_ref)
+    g_dev = dev.writes[0]["greenMatrixNum"] / dev.writes[0]["den"]
+    g_ref = ref.writes[0]["greenMatrixNum"] / ref.writes[0]["den"]
+    assert np.max(np.abs(g_dev - g_ref)) < 1e-4
+
+
+def green(left, right):
+    return (right @ np.linalg.solve(left.conj().T @ right, left.conj().T)).T
+
+
+@pytest.mark.parametrize("nbp,skip", [(30, 10), (6, 8)])
+def test_bp_u0_closed_form_larger(nbp, skip):
+    """U = 0, 2L = 128 dense orbitals, N = 24, 64 walkers: the back-propagated Green matrix is that of
+    exp(-nbp dt K) phiT against the lowest N orbitals of K, and the Wick correlators are its
+    products."""
+    rng = np.random.default_rng(5)
+    L, N, nw = 64, 24, 64
+    M = 2 * L
+    q, _ = np.linalg.qr(rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M)))
+    lam = np.concatenate([np.linspace(-3.0, -2.0, N), np.linspace(1.0, 3.0, M - N)])
+    K = (q * lam) @ q.conj().T
+    K = 0.5 * (K + K.conj().T)
+    model = HubbardModel(L=L, N=N, K=K, U=np.zeros(L))
+    trial, _ = np.linalg.qr(q[:, :N] + 0.5 * q[:, N:2 * N])
+    dt = 0.1
+    m = DevBP(dt=dt, thermalSize=150, writeNumber=1, measureNumberPerWrite=2, measureSkipStep=skip,
+              walkerSizePerThread=nw, decompType="densitySpin", mgsStep=5, popControlStep=7, ET=-3.0,
+              ETAdjustStep=10, ETAdjustMaxSize=0, backPropagationStep=nbp, seed=11)
+    dev = AfqmcConstraintPathBP(ctx(), model, m, trial, spin_blocks=False)
+    dev.run()
+    w = dev.writes[0]
+    ev, vec = np.linalg.eigh(model.K)
+    bra = (vec * np.exp(-nbp * dt * ev)) @ vec.conj().T @ trial
+    G = green(bra, vec[:, :N])
+    assert np.max(np.abs(w["greenMatrixNum"] / w["den"] - G)) < 1e-9
+    d = np.diag(G)
+    dd = np.outer(d, d) - G * G.T
+    np.fill_diagonal(dd, d)
+    assert np.max(np.abs(w["densityDensityNum"] / w["den"] - dd)) < 1e-9
+    sp = G[:L, :L] * G[L:, L:] - G[:L, L:] * G[L:, :L]
+    assert np.max(np.abs(w["spairSpairNum"] / w["den"] - sp)) < 1e-9
