@@ -315,6 +315,42 @@ def run_gradient_leg(ctx):
             "dE_dU": gU, "dE_dU_central_difference": fdU, "rel_diff": abs(gU - fdU) / max(1.0, abs(fdU))}
 
 
+def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed):
+    """Back-propagation on the benchmark workload (mixed precision): 40 thermal steps, a window of
+    nbp recorded steps, nmeas measurements every skip steps.  Each measurement is timed between
+    stream synchronisations; the estimates are the back-propagated energy and double occupancy."""
+    from afqmclab_amd.cp import AfqmcConstraintPathBP, BPMethod
+    Ls = model.L
+    m = BPMethod(dt=0.005, thermalSize=40, writeNumber=1, measureNumberPerWrite=nmeas, measureSkipStep=skip,
+                 walkerSizePerThread=nw, decompType="densitySpin", mgsStep=10, popControlStep=40, ETAdjustStep=10,
+                 ETAdjustMaxSize=40, backPropagationStep=nbp, seed=seed)
+    drv = AfqmcConstraintPathBP(ctx, model, m, phi, nup=nup, spin_blocks=True, precision=1)
+    times = []
+    measure = drv.bp.measure
+
+    def timed(w):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        measure(w)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+
+    drv.bp.measure = timed
+    t0 = time.perf_counter()
+    drv.run()
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    w = drv.writes[0]
+    dd = w["densityDensityNum"]
+    docc = sum(dd[i, i + Ls] for i in range(Ls)) / w["den"] / Ls
+    return {"workload": f"back-propagation, same lattice, trial and walkers, mixed precision, window "
+                        f"{nbp} steps, measurement every {skip} steps",
+            "measurements": len(times), "ms_per_measurement": 1e3 * float(np.mean(times)),
+            "bra_steps_per_measurement": nbp, "seconds_total": total,
+            "energy_per_site_bp": float((w["HNum"] / w["den"]).real / Ls),
+            "double_occupancy_bp": float(docc.real)}
+
+
 def cpu_baseline(model, phi, nw_cpu, budget_s, sched, seed):
     """Oracle (dense reference representation) on a bounded sample of the same workload."""
     from oracle.blocks import SD
@@ -358,6 +394,8 @@ def main():
     ap.add_argument("--ph-walkers", type=int, default=1024)
     ap.add_argument("--hetero-steps", type=int, default=200, help="heterogeneous batch leg steps (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="run the 4x4-vs-ED and 8x8 gradient legs (0 = skip)")
+    ap.add_argument("--bp-measurements", type=int, default=3,
+                    help="back-propagation leg: measurements with a 40-step window every 20 steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (reference kernel for HBM traffic counters)")
     args = ap.parse_args()
@@ -499,6 +537,11 @@ def main():
             "speedup": graph["walker_steps_per_s"] / eager["walker_steps_per_s"],
             "same_log_overlaps": bool(same),
         }
+    if args.bp_measurements > 0:
+        log(f"[bench] back-propagation: {args.walkers} walkers, window 40, {args.bp_measurements} measurements")
+        out["back_propagation"] = run_bp_leg(ctx, model, phi, nup, args.walkers, 40, 20, args.bp_measurements,
+                                             args.seed)
+        log(f"[bench]   back-propagation: {out['back_propagation']}")
     if args.extras:
         out["hubbard4x4_vs_exact"] = run_ed_check(ctx)
         log(f"[bench] 4x4 U=4 5+5: E = {out['hubbard4x4_vs_exact']['E']:.4f} +- "
