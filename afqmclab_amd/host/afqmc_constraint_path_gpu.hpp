@@ -33,6 +33,10 @@ inline void check(int rc, const char* what) {
 struct ConstraintPathMethod {
   double dt = 0.01;
   size_t thermalSize = 200, writeNumber = 60, measureNumberPerWrite = 2, measureSkipStep = 5;
+  // the back-propagation driver's extra key (tutorials/afqmcConstraintPathBP/source/
+  // afqmcConstraintPathMethod.cpp:26, default 100 at :166); read and written only when backPropagation
+  bool backPropagation = false;
+  size_t backPropagationStep = 100;
   int walkerSizePerThread = 300;
   std::string decompType = "densitySpin", forceType = "dynamicForce";
   double forceCap = 1.5;
@@ -49,6 +53,7 @@ struct ConstraintPathMethod {
     search_value(file, "writeNumber", writeNumber);
     search_value(file, "measureNumberPerWrite", measureNumberPerWrite);
     search_value(file, "measureSkipStep", measureSkipStep);
+    if (backPropagation) search_value(file, "backPropagationStep", backPropagationStep);
     search_value(file, "walkerSizePerThread", walkerSizePerThread);
     search_value(file, "decompType", decompType);
     search_value(file, "forceType", forceType);
@@ -70,7 +75,9 @@ struct ConstraintPathMethod {
     auto kv = [&](const char* k, const auto& v) { f << std::setw(36) << (std::string(k) + " ") << std::setw(26) << v << "\n"; };
     f << std::left << "\n";
     kv("dt", dt); kv("thermalSize", thermalSize); kv("writeNumber", writeNumber);
-    kv("measureNumberPerWrite", measureNumberPerWrite); kv("measureSkipStep", measureSkipStep); f << "\n";
+    kv("measureNumberPerWrite", measureNumberPerWrite); kv("measureSkipStep", measureSkipStep);
+    if (backPropagation) kv("backPropagationStep", backPropagationStep);
+    f << "\n";
     kv("walkerSizePerThread", walkerSizePerThread); f << "\n";
     kv("decompType", decompType); kv("forceType", forceType); kv("forceCap", forceCap);
     kv("initialPhiTFlag", initialPhiTFlag); kv("initialWalkerFlag", initialWalkerFlag); f << "\n";
@@ -136,8 +143,14 @@ class AfqmcConstraintPathGpu {
     check(afqmc_cp_walkers_create(cp, method.walkerSizePerThread, &walker), "afqmc_cp_walkers_create");
     check(afqmc_cp_set_seed(cp, (uint64_t)method.seed, 0), "afqmc_cp_set_seed");
     check(afqmc_cp_set_ET(cp, method.ET), "afqmc_cp_set_ET");
+    if (method.backPropagation) {
+      const int nbp = std::abs(method.dt) < 1e-12 ? 0 : (int)method.backPropagationStep;
+      check(afqmc_bp_create(cp, method.walkerSizePerThread, nbp, (int)method.measureSkipStep, (int)method.mgsStep, &bp),
+            "afqmc_bp_create");
+    }
   }
   ~AfqmcConstraintPathGpu() {
+    afqmc_bp_destroy(bp);
     afqmc_walkers_destroy(walker);
     afqmc_cp_destroy(cp);
     afqmc_ctx_destroy(ctx);
@@ -203,6 +216,7 @@ class AfqmcConstraintPathGpu {
 
   // afqmcConstraintPath.cpp:97-141
   void measureWithProjection() {
+    if (bp) return measureWithProjectionBP();
     projectExpMinusHalfDtK();
     size_t mgsIndex = 0, popControlIndex = 0;
     for (size_t i = 0; i < method.thermalSize; ++i) {
@@ -225,10 +239,48 @@ class AfqmcConstraintPathGpu {
       writeAndResetMeasurement();
     }
   }
-  // afqmcConstraintPath.cpp:89-96 (dt = 0)
+  // the back-propagation driver's loop (tutorials/afqmcConstraintPathBP/source/afqmcConstraintPath.cpp:115-180):
+  // every step after thermalisation is recorded on the device; addMeasurement fires where the
+  // reference's does, with its window bookkeeping (afqmcConstraintPathBackPropagation.cpp:59-76)
+  void measureWithProjectionBP() {
+    const size_t nbp = method.backPropagationStep, skip = method.measureSkipStep;
+    projectExpMinusHalfDtK();
+    size_t mgsIndex = 0, popControlIndex = 0;
+    for (size_t i = 0; i < method.thermalSize; ++i) {
+      if (i < method.ETAdjustMaxSize && i % method.ETAdjustStep == 0) {
+        addMixedMeasurement();
+        adjustETAndResetMeasurement();
+      }
+      projectOneStep(mgsIndex, popControlIndex);
+    }
+    prepareBackPropagation();
+    if (nbp >= skip)
+      for (size_t i = 0; i < nbp; ++i) projectOneStep(mgsIndex, popControlIndex);
+    for (size_t i = 0; i < method.writeNumber; ++i) {
+      for (size_t j = 0; j < method.measureNumberPerWrite; ++j) {
+        for (size_t k = 0; k < skip; ++k) {
+          if (BPIndex == nbp) addMeasurement();
+          projectOneStep(mgsIndex, popControlIndex);
+        }
+        if (nbp < skip) prepareBackPropagation();
+      }
+      if (!outDir.empty()) {
+        const double beta = (method.thermalSize + (i + 0.5) * method.measureNumberPerWrite * skip - 0.5) * method.dt;
+        refio::append(outDir + "/beta.dat", beta);
+      }
+      writeAndResetMeasurement();
+    }
+  }
+  // afqmcConstraintPath.cpp:89-96 (dt = 0); with back-propagation the measurement is the
+  // full-Green-matrix one of the mixed pair (a window of no steps)
   void measureWithoutProjection() {
     projectExpMinusHalfDtK();
-    addMixedMeasurement();
+    if (bp) {
+      check(afqmc_bp_begin(bp), "afqmc_bp_begin");
+      check(afqmc_bp_measure(bp, walker), "afqmc_bp_measure");
+    } else {
+      addMixedMeasurement();
+    }
     writeAndResetMeasurement();
   }
   // afqmcConstraintPath.cpp:149-155: the adjusted ET goes back into the method, the walkers are
@@ -249,15 +301,37 @@ class AfqmcConstraintPathGpu {
   void projectExpMinusHalfDtK() { check(afqmc_cp_onebody(cp, walker, 1), "afqmc_cp_onebody"); }
   void projectExpMinusDtKExpMinusDtV() { check(afqmc_cp_step(cp, walker, nullptr, nullptr), "afqmc_cp_step"); }
   void projectOneStep(size_t& mgsIndex, size_t& popControlIndex) {
-    projectExpMinusDtKExpMinusDtV();
+    if (recording) check(afqmc_bp_step(bp, walker), "afqmc_bp_step");
+    else projectExpMinusDtKExpMinusDtV();
     if (++mgsIndex == method.mgsStep) { modifyGM(); mgsIndex = 0; }
     if (++popControlIndex == method.popControlStep) { popControl(); popControlIndex = 0; }
+    if (isBP) ++BPIndex;
   }
   void modifyGM() { check(afqmc_cp_stabilize(cp, walker), "afqmc_cp_stabilize"); }
-  void popControl() { check(afqmc_cp_pop_control(cp, walker, nullptr, nullptr, nullptr), "afqmc_cp_pop_control"); }
+  void popControl() {
+    if (recording) check(afqmc_bp_pop_control(bp, walker, nullptr), "afqmc_bp_pop_control");
+    else check(afqmc_cp_pop_control(cp, walker, nullptr, nullptr, nullptr), "afqmc_cp_pop_control");
+  }
+  // afqmcConstraintPathBackPropagation.cpp:11-20: the device records from the first call on
+  void prepareBackPropagation() {
+    if (!recording) {
+      check(afqmc_bp_begin(bp), "afqmc_bp_begin");
+      recording = true;
+    }
+    isBP = true;
+    BPIndex = 0;
+  }
+  // afqmcConstraintPathMeasure.cpp:41-72
+  void addMeasurement() {
+    if (!isBP) return;
+    check(afqmc_bp_measure(bp, walker), "afqmc_bp_measure");
+    if (method.backPropagationStep >= method.measureSkipStep) BPIndex = method.backPropagationStep - method.measureSkipStep;
+    else isBP = false;
+  }
   void addMixedMeasurement() { check(afqmc_cp_measure(cp, walker), "afqmc_cp_measure"); }
   void adjustETAndResetMeasurement() { check(afqmc_cp_adjust_ET(cp), "afqmc_cp_adjust_ET"); }
   void writeAndResetMeasurement() {
+    if (bp) return writeAndResetMeasurementBP();
     double e[14];
     check(afqmc_cp_energy(cp, e), "afqmc_cp_energy");
     blocks.push_back({{e[2], e[3]}, {e[0], e[1]}, {e[4], e[5]}, {e[6], e[7]}});
@@ -287,6 +361,30 @@ class AfqmcConstraintPathGpu {
     check(afqmc_cp_reset_measure(cp), "afqmc_cp_reset_measure");
   }
 
+  // HubbardSOCMeasureObserveSDSD::write (HubbardSOCMeasureObserveSDSD.cpp:47-57): den, HNum, then
+  // KNum, VNum, RNum, then the observable matrices column-major, one complex per line
+  void writeAndResetMeasurementBP() {
+    using cd = std::complex<double>;
+    const size_t L = (size_t)model.L;
+    std::vector<cd> sc(5), g(4 * L * L), dd(4 * L * L), pm(L * L), mp(L * L), sp(L * L);
+    auto d = [](std::vector<cd>& v) { return reinterpret_cast<double*>(v.data()); };
+    check(afqmc_bp_get(bp, d(sc), d(g), d(dd), d(pm), d(mp), d(sp)), "afqmc_bp_get");
+    blocks.push_back({sc[1], sc[0], sc[2], sc[3]});
+    if (!outDir.empty()) {
+      refio::append(outDir + "/den.dat", sc[0]);
+      refio::append(outDir + "/HNum.dat", sc[1]);
+      refio::append(outDir + "/KNum.dat", sc[2]);
+      refio::append(outDir + "/VNum.dat", sc[3]);
+      refio::append(outDir + "/RNum.dat", sc[4]);
+      refio::append(outDir + "/greenMatrixNum.dat", g);
+      refio::append(outDir + "/densityDensityNum.dat", dd);
+      refio::append(outDir + "/splusSminusNum.dat", pm);
+      refio::append(outDir + "/sminusSplusNum.dat", mp);
+      refio::append(outDir + "/spairSpairNum.dat", sp);
+    }
+    check(afqmc_bp_reset(bp), "afqmc_bp_reset");
+  }
+
   static const double* dp(const std::vector<std::complex<double>>& v) {
     return reinterpret_cast<const double*>(v.data());
   }
@@ -296,6 +394,9 @@ class AfqmcConstraintPathGpu {
   afqmc_ctx ctx = nullptr;
   afqmc_cp cp = nullptr;
   afqmc_walkers walker = nullptr;
+  afqmc_bp bp = nullptr;
+  bool recording = false, isBP = false;
+  size_t BPIndex = 0;
   std::vector<BlockEnergy> blocks;
   std::vector<std::complex<double>> initWf, initLogw, finalWf, finalLogw;
   std::vector<int> initAlive, finalAlive;

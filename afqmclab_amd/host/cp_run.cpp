@@ -1,6 +1,11 @@
 // afqmc_cp_run: the constraint-path driver as a C++ program on the GPU library.
 //
-// Usage: afqmc_cp_run <dir> [precision]
+// Usage: afqmc_cp_run [--bp] <dir> [precision]
+//
+// --bp runs the back-propagation driver of tutorials/afqmcConstraintPathBP on the same directory:
+// afqmc_param also holds backPropagationStep, and the measurement files are those of
+// HubbardSOCMeasureObserveSDSD (den, HNum, KNum, VNum, RNum, greenMatrixNum, densityDensityNum,
+// splusSminusNum, sminusSplusNum, spairSpairNum).
 //
 // A run directory in the reference's own formats (afqmcConstraintPath.cpp:14-60,
 // afqmcConstraintPathSetWalker.cpp), used when <dir>/model_param exists:
@@ -76,8 +81,9 @@ refio::SDFile read_sd_checked(const std::string& path, size_t rows, size_t cols)
   return s;
 }
 
-int run_reference_dir(const std::string& dir, int precision) {
+int run_reference_dir(const std::string& dir, int precision, bool back_propagation) {
   afqmc_gpu::ConstraintPathMethod method;
+  method.backPropagation = back_propagation;
   method.read(dir + "/afqmc_param");
   const refio::HubbardModelParam m = refio::read_model_param(dir + "/model_param");
   const int L = (int)m.L, N = (int)m.N, L2 = 2 * L;
@@ -227,14 +233,21 @@ int run_bin_dir(const std::string& dir, int precision) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2) {
-    std::cerr << "usage: afqmc_cp_run <dir> [precision]\n";
+  std::vector<std::string> args(argv + 1, argv + argc);
+  bool back_propagation = false;
+  if (!args.empty() && args[0] == "--bp") {
+    back_propagation = true;
+    args.erase(args.begin());
+  }
+  if (args.empty()) {
+    std::cerr << "usage: afqmc_cp_run [--bp] <dir> [precision]\n";
     return 2;
   }
-  const std::string dir = argv[1];
-  const int precision = argc > 2 ? std::atoi(argv[2]) : 0;
+  const std::string dir = args[0];
+  const int precision = args.size() > 1 ? std::atoi(args[1].c_str()) : 0;
   try {
-    if (refio::file_exists(dir + "/model_param")) return run_reference_dir(dir, precision);
+    if (refio::file_exists(dir + "/model_param")) return run_reference_dir(dir, precision, back_propagation);
+    if (back_propagation) throw std::runtime_error("--bp needs a reference run directory (model_param)");
     return run_bin_dir(dir, precision);
   } catch (const std::exception& e) {
     std::cerr << "afqmc_cp_run: " << e.what() << std::endl;

@@ -112,7 +112,8 @@ def read_values(path, skip=0):
 
 
 # afqmcConstraintPathMethod.cpp:48-89: the keys in the order the reference writes them
-METHOD_KEYS = [("dt", "thermalSize", "writeNumber", "measureNumberPerWrite", "measureSkipStep"),
+# (backPropagationStep only in the back-propagation driver's file, :62 of its method)
+METHOD_KEYS = [("dt", "thermalSize", "writeNumber", "measureNumberPerWrite", "measureSkipStep", "backPropagationStep"),
                ("walkerSizePerThread",),
                ("decompType", "forceType", "forceCap", "initialPhiTFlag", "initialWalkerFlag"),
                ("mgsStep", "popControlStep"),

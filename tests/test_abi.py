@@ -64,3 +64,12 @@ def test_host_driver_reports_missing_input(tmp_path):
     r = subprocess.run([EXE, str(tmp_path / "nothing")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "Error opening file" in r.stderr and "afqmc_param" in r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="host driver not built (run __graft_entry__.build())")
+def test_host_driver_back_propagation_needs_a_reference_directory(tmp_path):
+    import subprocess
+    r = subprocess.run([EXE, "--bp", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "--bp needs a reference run directory" in r.stderr
+    r = subprocess.run([EXE, "--bp"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
