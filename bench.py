@@ -315,16 +315,19 @@ def run_gradient_leg(ctx):
             "dE_dU": gU, "dE_dU_central_difference": fdU, "rel_diff": abs(gU - fdU) / max(1.0, abs(fdU))}
 
 
-def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed):
+def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed, tiles=None):
     """Back-propagation on the benchmark workload (mixed precision): 40 thermal steps, a window of
     nbp recorded steps, nmeas measurements every skip steps.  Each measurement is timed between
-    stream synchronisations; the estimates are the back-propagated energy and double occupancy."""
-    from afqmclab_amd.cp import AfqmcConstraintPathBP, BPMethod
+    stream synchronisations; the estimates are the back-propagated energy and double occupancy.
+    `tiles` (propagate, overlap, Green tile names) are the main path's tuned GEMM tiles."""
+    from afqmclab_amd.cp import AfqmcConstraintPathBP, BPMethod, CPPropagator
     Ls = model.L
     m = BPMethod(dt=0.005, thermalSize=40, writeNumber=1, measureNumberPerWrite=nmeas, measureSkipStep=skip,
                  walkerSizePerThread=nw, decompType="densitySpin", mgsStep=10, popControlStep=40, ETAdjustStep=10,
                  ETAdjustMaxSize=40, backPropagationStep=nbp, seed=seed)
     drv = AfqmcConstraintPathBP(ctx, model, m, phi, nup=nup, spin_blocks=True, precision=1)
+    if tiles:
+        drv.prop.set_tiles(*[CPPropagator.TILES.index(t) for t in tiles])
     times = []
     measure = drv.bp.measure
 
@@ -346,7 +349,7 @@ def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed):
     return {"workload": f"back-propagation, same lattice, trial and walkers, mixed precision, window "
                         f"{nbp} steps, measurement every {skip} steps",
             "measurements": len(times), "ms_per_measurement": 1e3 * float(np.mean(times)),
-            "bra_steps_per_measurement": nbp, "seconds_total": total,
+            "bra_steps_per_measurement": nbp, "seconds_total": total, "gemm_tiles": tiles,
             "energy_per_site_bp": float((w["HNum"] / w["den"]).real / Ls),
             "double_occupancy_bp": float(docc.real)}
 
@@ -540,7 +543,7 @@ def main():
     if args.bp_measurements > 0:
         log(f"[bench] back-propagation: {args.walkers} walkers, window 40, {args.bp_measurements} measurements")
         out["back_propagation"] = run_bp_leg(ctx, model, phi, nup, args.walkers, 40, 20, args.bp_measurements,
-                                             args.seed)
+                                             args.seed, results["mixed"]["tiles"] if "mixed" in results else None)
         log(f"[bench]   back-propagation: {out['back_propagation']}")
     if args.extras:
         out["hubbard4x4_vs_exact"] = run_ed_check(ctx)
