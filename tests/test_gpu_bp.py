@@ -65,10 +65,25 @@ def test_bp_spin_mixing_matches_oracle(nbp, skip):
     assert dev.betas == pytest.approx(ref.betas)
 
 
-@pytest.mark.parametrize("mode", ["dense", "blocks"])
-def test_bp_hubbard_matches_oracle(mode):
-    """4x4 Hubbard, densitySpin, diagonal site matrices: dense walkers and spin blocks."""
-    nx, ny, nup, ndn, U = 4, 4, 5, 5, 4.0
+@pytest.mark.parametrize("decomp", ["hopCharge", "hopSpin"])
+def test_bp_hop_decompositions_match_oracle(decomp):
+    """The hop decompositions mix the spins at every site with non-Hermitian 2x2 site matrices, so
+    the bras need their true adjoints."""
+    model = mixing_model(seed=8, U=3.0)
+    phi = free_electron_trial(model)
+    seed = 77
+    m = dict(dt=0.05, thermalSize=6, writeNumber=1, measureNumberPerWrite=3, measureSkipStep=3,
+             walkerSizePerThread=8, decompType=decomp, forceType="dynamicForce", mgsStep=2,
+             popControlStep=3, ET=-3.0, ETAdjustStep=3, ETAdjustMaxSize=6, backPropagationStep=5, seed=seed)
+    dev, ref = run_pair(model, phi, m, dict(spin_blocks=False), seed)
+    compare_writes(dev.writes, ref.writes, 1e-9)
+
+
+@pytest.mark.parametrize("mode,nup,ndn", [("dense", 5, 5), ("blocks", 5, 5), ("blocks", 6, 3)])
+def test_bp_hubbard_matches_oracle(mode, nup, ndn):
+    """4x4 Hubbard, densitySpin, diagonal site matrices: dense walkers and spin blocks (also with
+    Nup != Ndn)."""
+    nx, ny, U = 4, 4, 4.0
     model = HubbardModel.square(nx, ny, nup + ndn, U)
     phi = free_electron_trial(model) if mode == "dense" else spin_block_trial(model, nup, ndn)
     kw = dict(spin_blocks=False) if mode == "dense" else dict(nup=nup, spin_blocks=True)
