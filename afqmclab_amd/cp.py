@@ -245,7 +245,8 @@ class CPPropagator:
         return WalkerSet(self, nw)
 
     # ---- GEMM tile shapes -------------------------------------------------
-    TILES = ["64x64", "32x64", "64x32", "32x32", "128x64", "64x64k32", "64x32k32", "32x64k32"]
+    TILES = ["64x64", "32x64", "64x32", "32x32", "128x64", "64x64k32", "64x32k32", "32x64k32", "128x128", "256x64",
+             "64x128"]
 
     def set_tiles(self, propagate=0, overlap=0, green=0):
         t = np.array([propagate, overlap, green], dtype=np.int32)

@@ -960,7 +960,7 @@ int afqmc_cp_adjust_ET(afqmc_cp cp) {
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles) {
   REQUIRE(cp && tiles, "null argument");
   for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= 0 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
-  REQUIRE(gemm_tile_bn(tiles[2]) == 64, "the Green GEMM takes a 64-column tile (0, 1, 4 or 5)");
+  REQUIRE(gemm_tile_bn(tiles[2]) == 64, "the Green GEMM takes a 64-column tile (0, 1, 4, 5, 7 or 9)");
   for (int r = 0; r < 3; ++r) cp->tile[r] = tiles[r];
   return 0;
 }

@@ -79,9 +79,23 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
       acc_x[i][j] = acc_t{0, 0, 0, 0};
     }
 
+  // The next K tile is loaded into registers before the current one is multiplied and stored to
+  // LDS after it: loads are issued unconditionally (out-of-range lanes read the operand's first
+  // element) and masked, and the HS scale is applied at the store, so no wait on the loads sits
+  // between their issue and the MFMAs.
   ct ra[A_ELEMS], rb[B_ELEMS];
+  ct rs[SCALE ? B_ELEMS : 1];
+  bool ina[A_ELEMS], inb[B_ELEMS];
+  const bool kpos = p.K > 0;
 
   auto load_tiles = [&](int k0) {
+    if (!kpos) {
+#pragma unroll
+      for (int e = 0; e < A_ELEMS; ++e) ina[e] = false;
+#pragma unroll
+      for (int e = 0; e < B_ELEMS; ++e) inb[e] = false;
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < A_ELEMS; ++e) {
       int idx = tid + e * NT;
@@ -89,27 +103,23 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
       if (OPA == OP_N || OPA == OP_NR) { m = idx % BM; k = idx / BM; }
       else { k = idx % BK; m = idx / BK; }
       int gm = m0 + m, gk = k0 + k;
-      ct v = cmk<T>(0, 0);
-      if (gm < p.M && gk < p.K) {
-        if (OPA == OP_NR) v = cmk<T>(Ar[gm + (long long)gk * p.lda], (T)0);
-        else v = (OPA == OP_N) ? A[gm + (long long)gk * p.lda] : A[gk + (long long)gm * p.lda];
-      }
-      ra[e] = v;
+      const bool in = gm < p.M && gk < p.K;
+      ina[e] = in;
+      const long long off = in ? ((OPA == OP_N || OPA == OP_NR) ? gm + (long long)gk * p.lda
+                                                                : gk + (long long)gm * p.lda)
+                               : 0;
+      if (OPA == OP_NR) ra[e] = cmk<T>(Ar[off], (T)0);
+      else ra[e] = A[off];
     }
 #pragma unroll
     for (int e = 0; e < B_ELEMS; ++e) {
       int idx = tid + e * NT;
       int k = idx % BK, n = idx / BK;
       int gk = k0 + k, gn = n0 + n;
-      ct v = cmk<T>(0, 0);
-      if (gk < p.K && gn < p.N) {
-        v = Bm[gk + (long long)gn * p.ldb];
-        if (SCALE) {
-          ct s = S[(long long)(gn / p.scale_group) * p.scale_strideG + gk];
-          v = cmul(s, v);
-        }
-      }
-      rb[e] = v;
+      const bool in = gk < p.K && gn < p.N;
+      inb[e] = in;
+      rb[e] = Bm[in ? gk + (long long)gn * p.ldb : 0];
+      if (SCALE) rs[e] = S[in ? (long long)(gn / p.scale_group) * p.scale_strideG + gk : 0];
     }
   };
   auto store_tiles = [&]() {
@@ -119,15 +129,18 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
       int m, k;
       if (OPA == OP_N || OPA == OP_NR) { m = idx % BM; k = idx / BM; }
       else { k = idx % BK; m = idx / BK; }
-      As[0][k][m] = ra[e].re;
-      As[1][k][m] = (OPA == OP_C) ? -ra[e].im : ra[e].im;
+      const ct v = ina[e] ? ra[e] : cmk<T>(0, 0);
+      As[0][k][m] = v.re;
+      As[1][k][m] = (OPA == OP_C) ? -v.im : v.im;
     }
 #pragma unroll
     for (int e = 0; e < B_ELEMS; ++e) {
       int idx = tid + e * NT;
       int k = idx % BK, n = idx / BK;
-      Bs[0][k][n] = rb[e].re;
-      Bs[1][k][n] = rb[e].im;
+      ct v = cmk<T>(0, 0);
+      if (inb[e]) v = SCALE ? cmul(rs[e], rb[e]) : rb[e];
+      Bs[0][k][n] = v.re;
+      Bs[1][k][n] = v.im;
     }
   };
 
@@ -227,6 +240,7 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
     // this tile's columns, written as partial sums per column tile.
     const ct* P0 = reinterpret_cast<const ct*>(p.P0);
     const ct* P2 = reinterpret_cast<const ct*>(p.P2);
+    const bool use1 = p.partner_half != 0, use2 = P2 != nullptr;
     __shared__ zd red[3][WN][BM];
     zd d0[FM][4], d1[FM][4], d2[FM][4];
 #pragma unroll
@@ -237,11 +251,57 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
         d1[i][r] = cmk<double>(0, 0);
         d2[i][r] = cmk<double>(0, 0);
       }
+    // FP32 accumulators: lane (lk, lr) holds rows 4 lk .. 4 lk + 3 of column lr, which are
+    // contiguous in a column-major partner: two 16-byte loads instead of four scattered ones
+    constexpr bool ROWS4 = std::is_same<T, float>::value;
+    const bool vec_ok = ROWS4 && (p.ldp % 2 == 0);
+    const bool vec_partner = vec_ok && (p.partner_half % 4 == 0);
+    auto load4 = [&](const ct* P, int gr0, int gn, bool fast, ct* q) {
+      const ct* src = P + gr0 + (long long)gn * p.ldp;
+      if (fast) {
+        const float4 a = *reinterpret_cast<const float4*>(src);
+        const float4 c = *reinterpret_cast<const float4*>(src + 2);
+        q[0] = cmk<T>((T)a.x, (T)a.y); q[1] = cmk<T>((T)a.z, (T)a.w);
+        q[2] = cmk<T>((T)c.x, (T)c.y); q[3] = cmk<T>((T)c.z, (T)c.w);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r] = src[r];
+      }
+    };
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         int gn = n0 + wn * TN + j * 16 + lr;
+        if (ROWS4) {
+          const int gm0 = m0 + wm * TM + i * 16 + 4 * lk;
+          if (gn < p.N && gm0 + 3 < p.M) {
+            ct q0[4], q1[4], q2[4];
+            load4(P0, gm0, gn, vec_ok, q0);
+            if (use1) {
+              int gp0 = gm0 < p.partner_half ? gm0 + p.partner_half : gm0 - p.partner_half;
+              if (vec_partner) {
+                load4(P0, gp0, gn, true, q1);
+              } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  int gm = gm0 + r;
+                  int gp = gm < p.partner_half ? gm + p.partner_half : gm - p.partner_half;
+                  q1[r] = P0[gp + (long long)gn * p.ldp];
+                }
+              }
+            }
+            if (use2) load4(P2, gm0, gn, vec_ok, q2);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              zd x = cmk<double>((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
+              d0[i][r] = cadd(d0[i][r], cmul(x, cconj(to_zd(q0[r]))));
+              if (use1) d1[i][r] = cadd(d1[i][r], cmul(x, cconj(to_zd(q1[r]))));
+              if (use2) d2[i][r] = cadd(d2[i][r], cmul(x, to_zd(q2[r])));
+            }
+            continue;
+          }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           int gm = m0 + wm * TM + i * 16 + mfma_traits<T>::row(lk, r);
@@ -249,19 +309,19 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
             zd x = cmk<double>((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
             zd q0 = to_zd(P0[gm + (long long)gn * p.ldp]);
             d0[i][r] = cadd(d0[i][r], cmul(x, cconj(q0)));
-            if (p.partner_half) {
+            if (use1) {
               int gp = gm < p.partner_half ? gm + p.partner_half : gm - p.partner_half;
               zd q1 = to_zd(P0[gp + (long long)gn * p.ldp]);
               d1[i][r] = cadd(d1[i][r], cmul(x, cconj(q1)));
             }
-            if (P2) {
+            if (use2) {
               zd q2 = to_zd(P2[gm + (long long)gn * p.ldp]);
               d2[i][r] = cadd(d2[i][r], cmul(x, q2));
             }
           }
         }
       }
-    // reduce over the 16 lanes that share a row (lane % 16 = column)
+    // reduce over the 16 lanes that share a row (lane % 16 = column); only the dots in use
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -270,10 +330,20 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
         for (int o = 1; o < 16; o <<= 1) {
           d0[i][r].re += __shfl_xor(d0[i][r].re, o, 64);
           d0[i][r].im += __shfl_xor(d0[i][r].im, o, 64);
-          d1[i][r].re += __shfl_xor(d1[i][r].re, o, 64);
-          d1[i][r].im += __shfl_xor(d1[i][r].im, o, 64);
-          d2[i][r].re += __shfl_xor(d2[i][r].re, o, 64);
-          d2[i][r].im += __shfl_xor(d2[i][r].im, o, 64);
+        }
+        if (use1) {
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) {
+            d1[i][r].re += __shfl_xor(d1[i][r].re, o, 64);
+            d1[i][r].im += __shfl_xor(d1[i][r].im, o, 64);
+          }
+        }
+        if (use2) {
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) {
+            d2[i][r].re += __shfl_xor(d2[i][r].re, o, 64);
+            d2[i][r].im += __shfl_xor(d2[i][r].im, o, 64);
+          }
         }
         if (lr == 0) {
           int m = wm * TM + i * 16 + mfma_traits<T>::row(lk, r);
@@ -315,7 +385,7 @@ int run(const GemmArgs& a, hipStream_t s) {
 
 template <typename T, int OPA, int EPI, bool SCALE>
 int dispatch_tile(const GemmArgs& a, hipStream_t s) {
-  // 4 waves (2 x 2), K tile 16; 64 x 64 holds 32 accumulator values per lane
+  // 4 waves (2 x 2) unless the shape says 8; K tile 16 unless K32
   switch (a.tile) {
     case GEMM_TILE_32x64: return run<T, OPA, 32, 64, 16, 2, 2, EPI, SCALE>(a, s);
     case GEMM_TILE_64x32: return run<T, OPA, 64, 32, 16, 2, 2, EPI, SCALE>(a, s);
@@ -324,6 +394,9 @@ int dispatch_tile(const GemmArgs& a, hipStream_t s) {
     case GEMM_TILE_64x64_K32: return run<T, OPA, 64, 64, 32, 2, 2, EPI, SCALE>(a, s);
     case GEMM_TILE_64x32_K32: return run<T, OPA, 64, 32, 32, 2, 2, EPI, SCALE>(a, s);
     case GEMM_TILE_32x64_K32: return run<T, OPA, 32, 64, 32, 2, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_128x128: return run<T, OPA, 128, 128, 16, 2, 4, EPI, SCALE>(a, s);
+    case GEMM_TILE_256x64: return run<T, OPA, 256, 64, 16, 4, 2, EPI, SCALE>(a, s);
+    case GEMM_TILE_64x128: return run<T, OPA, 64, 128, 16, 2, 2, EPI, SCALE>(a, s);
     default: return run<T, OPA, 64, 64, 16, 2, 2, EPI, SCALE>(a, s);
   }
 }
@@ -340,6 +413,7 @@ int dispatch_green(const GemmArgs& a, hipStream_t s) {
   if (a.tile == GEMM_TILE_128x64) return run<T, OP_N, 128, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
   if (a.tile == GEMM_TILE_64x64_K32) return run<T, OP_N, 64, 64, 32, 2, 2, EPI_GREEN, false>(a, s);
   if (a.tile == GEMM_TILE_32x64_K32) return run<T, OP_N, 32, 64, 32, 2, 2, EPI_GREEN, false>(a, s);
+  if (a.tile == GEMM_TILE_256x64) return run<T, OP_N, 256, 64, 16, 4, 2, EPI_GREEN, false>(a, s);
   return run<T, OP_N, 64, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
 }
 

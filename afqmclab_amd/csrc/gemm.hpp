@@ -42,16 +42,21 @@ struct GemmArgs {
   int herm, b_upper;
 };
 
-// Output tile shapes of the MFMA GEMM (4 waves each): the autotuner picks one per call site.
+// Output tile shapes of the MFMA GEMM: the autotuner picks one per call site.
 enum {
   GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_64x32 = 2, GEMM_TILE_32x32 = 3, GEMM_TILE_128x64 = 4,
-  GEMM_TILE_64x64_K32 = 5, GEMM_TILE_64x32_K32 = 6, GEMM_TILE_32x64_K32 = 7, GEMM_TILE_COUNT = 8
+  GEMM_TILE_64x64_K32 = 5, GEMM_TILE_64x32_K32 = 6, GEMM_TILE_32x64_K32 = 7,
+  // 8 waves (128x128: 2 x 4, 256x64: 4 x 2) and a 64x128 tile of 4 waves, all with a 16-deep K tile
+  GEMM_TILE_128x128 = 8, GEMM_TILE_256x64 = 9, GEMM_TILE_64x128 = 10, GEMM_TILE_COUNT = 11
 };
 inline int gemm_tile_bn(int tile) {
-  return (tile == GEMM_TILE_64x32 || tile == GEMM_TILE_32x32 || tile == GEMM_TILE_64x32_K32) ? 32 : 64;
+  if (tile == GEMM_TILE_64x32 || tile == GEMM_TILE_32x32 || tile == GEMM_TILE_64x32_K32) return 32;
+  if (tile == GEMM_TILE_128x128 || tile == GEMM_TILE_64x128) return 128;
+  return 64;
 }
 inline bool gemm_tile_square(int tile) {
-  return tile == GEMM_TILE_64x64 || tile == GEMM_TILE_32x32 || tile == GEMM_TILE_64x64_K32;
+  return tile == GEMM_TILE_64x64 || tile == GEMM_TILE_32x32 || tile == GEMM_TILE_64x64_K32 ||
+         tile == GEMM_TILE_128x128;
 }
 
 template <typename T>
