@@ -309,7 +309,7 @@ int tune_role(afqmc_cp cp, afqmc_walkers w, int role, int reps, double* ms_row) 
   int rc = 0;
   for (int t = 0; t < GEMM_TILE_COUNT && !rc; ++t) {
     ms_row[t] = -1.0;
-    if (role == 2 && gemm_tile_bn(t) != 64) continue;
+    if (role == 2 && !gemm_tile_green(t)) continue;
     cp->tile[role] = t;
     auto run = [&]() {
       // each role reads the walkers and writes only workspace (tmp, overlaps, Green partials)
@@ -960,7 +960,7 @@ int afqmc_cp_adjust_ET(afqmc_cp cp) {
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles) {
   REQUIRE(cp && tiles, "null argument");
   for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= 0 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
-  REQUIRE(gemm_tile_bn(tiles[2]) == 64, "the Green GEMM takes a 64-column tile (0, 1, 4, 5, 7 or 9)");
+  REQUIRE(gemm_tile_green(tiles[2]), "the Green GEMM takes tile shape 0, 1, 4, 5, 7, 8 or 9");
   for (int r = 0; r < 3; ++r) cp->tile[r] = tiles[r];
   return 0;
 }

@@ -353,20 +353,29 @@ __global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
         }
       }
     __syncthreads();
-    const long long ntn = gridDim.y;
+    // one partial sum per 64-column group (two waves of 32 columns each, summed in wave order), so
+    // that every tile shape adds the same terms in the same order
+    static_assert(EPI != EPI_GREEN || (TN == 32 && BN % 64 == 0), "Green tiles: 32 columns per wave");
+    constexpr int GROUPS = BN / 64, WPG = 64 / TN;
+    const int ngroups = (p.N + 63) / 64;
     for (int m = tid; m < BM; m += NT) {
       int gm = m0 + m;
       if (gm >= p.M) continue;
-      zd s0 = cmk<double>(0, 0), s1 = s0, s2 = s0;
-      for (int w = 0; w < WN; ++w) {
-        s0 = cadd(s0, red[0][w][m]);
-        s1 = cadd(s1, red[1][w][m]);
-        s2 = cadd(s2, red[2][w][m]);
+#pragma unroll
+      for (int q = 0; q < GROUPS; ++q) {
+        const int slot = by * GROUPS + q;
+        if (slot >= ngroups) break;
+        zd s0 = cmk<double>(0, 0), s1 = s0, s2 = s0;
+        for (int w = q * WPG; w < (q + 1) * WPG; ++w) {
+          s0 = cadd(s0, red[0][w][m]);
+          s1 = cadd(s1, red[1][w][m]);
+          s2 = cadd(s2, red[2][w][m]);
+        }
+        long long off = ((long long)b * ngroups + slot) * p.M + gm;
+        p.out0[off] = s0;
+        if (p.out1) p.out1[off] = s1;
+        if (p.out2) p.out2[off] = s2;
       }
-      long long off = ((long long)b * ntn + by) * p.M + gm;
-      p.out0[off] = s0;
-      if (p.out1) p.out1[off] = s1;
-      if (p.out2) p.out2[off] = s2;
     }
   }
 }
@@ -414,6 +423,7 @@ int dispatch_green(const GemmArgs& a, hipStream_t s) {
   if (a.tile == GEMM_TILE_64x64_K32) return run<T, OP_N, 64, 64, 32, 2, 2, EPI_GREEN, false>(a, s);
   if (a.tile == GEMM_TILE_32x64_K32) return run<T, OP_N, 32, 64, 32, 2, 2, EPI_GREEN, false>(a, s);
   if (a.tile == GEMM_TILE_256x64) return run<T, OP_N, 256, 64, 16, 4, 2, EPI_GREEN, false>(a, s);
+  if (a.tile == GEMM_TILE_128x128) return run<T, OP_N, 128, 128, 16, 2, 4, EPI_GREEN, false>(a, s);
   return run<T, OP_N, 64, 64, 16, 2, 2, EPI_GREEN, false>(a, s);
 }
 
@@ -427,7 +437,7 @@ int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s) {
   REQUIRE(!sc || a.scale_group > 0, "cgemm: scale_group must be positive");
   REQUIRE(epi == EPI_STORE || (a.out0 && a.P0), "cgemm: green epilogue needs P0/out0");
   REQUIRE(a.tile >= 0 && a.tile < GEMM_TILE_COUNT, "cgemm: unknown tile shape %d", a.tile);
-  REQUIRE(epi != EPI_GREEN || gemm_tile_bn(a.tile) == 64, "cgemm: the Green epilogue needs a BN = 64 tile");
+  REQUIRE(epi != EPI_GREEN || gemm_tile_green(a.tile), "cgemm: tile shape %d has no Green epilogue", a.tile);
   REQUIRE(!a.herm || (opa == OP_C && epi == EPI_STORE && !a.beta_one && a.alpha_im == 0.0 && a.M == a.N &&
                       gemm_tile_square(a.tile)),
           "cgemm: a Hermitian result needs op C, a square tile, M == N, real alpha and a plain store");

@@ -27,13 +27,13 @@ struct GemmArgs {
   const void* P2;          // (phiT^H K)^T: d2 = sum_j X(i,j) P2(i,j)
   long long ldp;
   int partner_half;        // i^ = i +- partner_half for the off-diagonal pairs (0 = none)
-  zd* out0; zd* out1; zd* out2;  // [batch][ntileN][M] partial sums
+  zd* out0; zd* out1; zd* out2;  // [batch][ceil(N / 64)][M] partial sums, one per 64-column group
   // alive mask: batches with alive[b]==0 are skipped (batched mode only); nullptr = all
   const int* alive;
   // tile order within an XCD: 0 = M tiles fastest (B panels shared), 1 = N tiles fastest (A panels
   // shared; for a large shared A against many narrow B panels)
   int n_fastest;
-  // output tile shape (GEMM_TILE_*); the Green epilogue supports the BN = 64 shapes only
+  // output tile shape (GEMM_TILE_*); the Green epilogue takes the shapes of gemm_tile_green
   int tile;
   // herm: the result is Hermitian (a Gram matrix op(A) B with op C and A = B): only the tiles on
   // or above the diagonal are computed, the others written as their conjugate transposes
@@ -54,6 +54,8 @@ inline int gemm_tile_bn(int tile) {
   if (tile == GEMM_TILE_128x128 || tile == GEMM_TILE_64x128) return 128;
   return 64;
 }
+// shapes with the Green epilogue: 32 columns per wave and whole 64-column groups
+inline bool gemm_tile_green(int tile) { return gemm_tile_bn(tile) == 64 || tile == GEMM_TILE_128x128; }
 inline bool gemm_tile_square(int tile) {
   return tile == GEMM_TILE_64x64 || tile == GEMM_TILE_32x32 || tile == GEMM_TILE_64x64_K32 ||
          tile == GEMM_TILE_128x128;

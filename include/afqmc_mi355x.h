@@ -122,7 +122,7 @@ int afqmc_cp_get_ET(afqmc_cp cp, double* ET);
  * tiles[1] the overlap, tiles[2] the Green's function GEMM; 0 = 64x64, 1 = 32x64, 2 = 64x32,
  * 3 = 32x32, 4 = 128x64 output tiles with a 16-deep K tile, 5 = 64x64, 6 = 64x32, 7 = 32x64 with a
  * 32-deep K tile, 8 = 128x128 and 9 = 256x64 (8 waves), 10 = 64x128, with a 16-deep K tile (the Green
- * GEMM takes the 64-column shapes 0, 1, 4, 5, 7, 9).  autotune times every shape on the walker set's
+ * GEMM takes shapes 0, 1, 4, 5, 7, 8 and 9).  autotune times every shape on the walker set's
  * own buffers (reps launches each), keeps the fastest and returns the choices in tiles (3) and the
  * times in ms (3 x 11, ms per launch, -1 = not applicable).  Every shape gives bit-identical results. */
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles);

@@ -13,7 +13,8 @@ from tests.gpu_util import ctx  # noqa: E402
 
 
 # the Green GEMM's epilogue needs the 64-column tiles
-GREEN_TILES = tuple(t for t, name in enumerate(CPPropagator.TILES) if name.split("x")[1].startswith("64"))
+GREEN_TILES = tuple(t for t, name in enumerate(CPPropagator.TILES)
+                    if name.split("x")[1].startswith("64") or name == "128x128")
 
 
 def make(c, nx, ny, nup, U, precision=0, nw=16, seed=5, t2=0.0, blocks=True):
