@@ -474,9 +474,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if (tr.get("workload_walkers") == args.walkers and tr.get("path") == main_path
-                and tr.get("tile") == r["tiles"][0]):
-            out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+        by_tile = tr.get("hbm_bytes_per_launch_by_tile", {tr.get("tile"): tr.get("hbm_bytes_per_launch")})
+        if tr.get("workload_walkers") == args.walkers and tr.get("path") == main_path and r["tiles"][0] in by_tile:
+            out["roofline"]["traffic"] = by_tile[r["tiles"][0]]
             out["roofline"]["algorithmic_bytes"] = tr.get("algorithmic_bytes_per_launch")
     if copy_cal:
         out["copy_calibration"] = copy_cal
