@@ -148,3 +148,36 @@ def test_bp_u0_closed_form_larger(nbp, skip):
     assert np.max(np.abs(w["densityDensityNum"] / w["den"] - dd)) < 1e-9
     sp = G[:L, :L] * G[L:, L:] - G[:L, L:] * G[L:, :L]
     assert np.max(np.abs(w["spairSpairNum"] / w["den"] - sp)) < 1e-9
+
+
+def test_bp_errors_are_reported():
+    """Calls out of schedule return an error with a message instead of touching the device."""
+    from afqmclab_amd._lib import AfqmcError
+    from afqmclab_amd.cp import BackPropagation, CPPropagator, CPMethod
+    model = HubbardModel.square(2, 2, 4, 4.0)
+    phi = spin_block_trial(model, 2, 2)
+    prop = CPPropagator(ctx(), model, CPMethod(dt=0.05, walkerSizePerThread=4), phi, nup=2, spin_blocks=True)
+    w = prop.walkers(4)
+    w.set_from_trial(phi)
+    bp = BackPropagation(prop, 4, 6, 3, 2)
+    with pytest.raises(AfqmcError, match="before afqmc_bp_begin"):
+        bp.step(w)
+    bp.begin()
+    with pytest.raises(AfqmcError, match="table of the last"):
+        bp.pop_control(w)
+    bp.step(w)
+    with pytest.raises(AfqmcError, match="recorded steps"):
+        bp.measure(w)
+    for _ in range(6):
+        bp.step(w)
+    # step 7: the window [1, 7) does not start at a multiple of measureSkipStep
+    with pytest.raises(AfqmcError, match="no walker copy"):
+        bp.measure(w)
+    bp.step(w)
+    bp.step(w)
+    bp.measure(w)  # window [3, 9)
+    assert bp.steps() == (9, 1)
+    assert np.isfinite(bp.get()["den"])
+    w5 = prop.walkers(3)
+    with pytest.raises(AfqmcError):
+        bp.step(w5)
