@@ -17,7 +17,8 @@ of all walkers plus QR / comb / measurement at their cadence.
 
 cpu_baseline: the CPU oracle (numpy + LAPACK restatement of the reference
 algorithm, dense 2Ls x N walkers as the reference runs them) timed on a
-bounded sample of the same workload on this host.
+bounded sample of the same workload on this host: one worker process per core
+with single-threaded BLAS, as the reference runs one MPI rank per core.
 """
 import argparse
 import json
@@ -354,16 +355,19 @@ def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed, tiles=None):
             "double_occupancy_bp": float(docc.real)}
 
 
-def cpu_baseline(model, phi, nw_cpu, budget_s, sched, seed):
-    """Oracle (dense reference representation) on a bounded sample of the same workload."""
+def cpu_worker(argv):
+    """One walker stream of the CPU baseline (run as `bench.py --cpu-worker DIR RANK WALKERS SECONDS`):
+    the oracle's CP step on dense 2Ls x N walkers with single-threaded BLAS, timed after set-up."""
+    d, rank, nw, budget = argv[0], int(argv[1]), int(argv[2]), float(argv[3])
     from oracle.blocks import SD
     from oracle.drivers import AfqmcConstraintPath as OracleCP, CPMethod as OMethod
     from oracle.models import HubbardSOC
     from oracle.rng import UniformStream
-    om = HubbardSOC(model.L, model.N, model.K, U=model.U)
-    m = OMethod(dt=sched["dt"], walkerSizePerThread=nw_cpu, decompType="densitySpin", mgsStep=sched["mgs"],
-                popControlStep=sched["pop"], thermalSize=0, ETAdjustMaxSize=0, ET=-10.0)
-    drv = OracleCP(m, om, UniformStream(seed), phiT=SD(wf=phi))
+    z = np.load(os.path.join(d, "inputs.npz"), allow_pickle=False)
+    om = HubbardSOC(int(z["L"]), int(z["N"]), z["K"], U=z["U"])
+    m = OMethod(dt=float(z["dt"]), walkerSizePerThread=nw, decompType="densitySpin", mgsStep=int(z["mgs"]),
+                popControlStep=int(z["pop"]), thermalSize=0, ETAdjustMaxSize=0, ET=-10.0)
+    drv = OracleCP(m, om, UniformStream(int(z["seed"]) + rank), phiT=SD(wf=z["phi"]))
     drv.initialWalker()
     drv.estimateMemory()
     drv.projectExpMinusHalfDtK()
@@ -373,13 +377,40 @@ def cpu_baseline(model, phi, nw_cpu, budget_s, sched, seed):
     while True:
         drv.projectOneStep(idx)
         nsteps += 1
-        if time.perf_counter() - t0 > budget_s:
+        if time.perf_counter() - t0 > budget:
             break
-    dt = time.perf_counter() - t0
-    return nsteps * nw_cpu / dt, nsteps
+    print(json.dumps({"steps": nsteps, "seconds": time.perf_counter() - t0, "walkers": nw}), flush=True)
+    return 0
+
+
+def cpu_baseline(model, phi, nw_per, budget_s, sched, seed):
+    """The oracle (dense reference representation) on a bounded sample of the same workload, one
+    walker stream per host core with single-threaded BLAS (the reference runs one MPI rank per core):
+    as many worker processes as cores, each with nw_per walkers and its own comb, all timed
+    concurrently; the rate is the sum of the workers' walker-steps/s."""
+    import subprocess
+    import tempfile
+    cores = len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    with tempfile.TemporaryDirectory() as d:
+        np.savez(os.path.join(d, "inputs.npz"), L=model.L, N=model.N, K=model.K, U=model.U, phi=phi,
+                 dt=sched["dt"], mgs=sched["mgs"], pop=sched["pop"], seed=seed)
+        procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", d, str(r), str(nw_per),
+                                   str(budget_s)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+                 for r in range(cores)]
+        outs = []
+        for pr in procs:
+            out, err = pr.communicate(timeout=budget_s * 4 + 300)
+            if pr.returncode != 0:
+                raise RuntimeError(f"cpu worker failed: {err[-2000:]}")
+            outs.append(json.loads(out.strip().splitlines()[-1]))
+    rate = sum(o["steps"] * o["walkers"] / o["seconds"] for o in outs)
+    return rate, cores, min(o["steps"] for o in outs)
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
+        return cpu_worker(sys.argv[2:])
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
@@ -391,7 +422,7 @@ def main():
     ap.add_argument("--seed", type=int, default=985456376)
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-walkers", type=int, default=16)
+    ap.add_argument("--cpu-walkers", type=int, default=2, help="walkers per CPU worker process")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ph-steps", type=int, default=60, help="phaseless leg steps (0 = skip)")
     ap.add_argument("--ph-walkers", type=int, default=1024)
@@ -552,15 +583,17 @@ def main():
         out["gradient_8x8"] = run_gradient_leg(ctx)
         log(f"[bench] 8x8 fixed-field gradient: {out['gradient_8x8']}")
     if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle, {args.cpu_walkers} walkers, ~{args.cpu_seconds}s")
-        ws_s, nst = cpu_baseline(model, phi, args.cpu_walkers, args.cpu_seconds, sched, args.seed)
-        cores = len(os.sched_getaffinity(0))
+        log(f"[bench] cpu baseline: oracle, one process per core, {args.cpu_walkers} walkers each, "
+            f"~{args.cpu_seconds}s")
+        ws_s, cores, nst = cpu_baseline(model, phi, args.cpu_walkers, args.cpu_seconds, sched, args.seed)
         out["cpu_baseline"] = {"value": ws_s, "unit": "walker-steps/s", "cores": cores, "kind": "port",
-                               "sample": f"{nst} steps x {args.cpu_walkers} walkers of the same workload, dense "
-                                         f"{2 * Ls}x{2 * nup} walkers (numpy/LAPACK oracle, BLAS threads = cores)"}
+                               "sample": f"{cores} processes (one per core, single-threaded OpenBLAS) x "
+                                         f"{args.cpu_walkers} walkers, >= {nst} steps each in ~{args.cpu_seconds:.0f} s, "
+                                         f"dense {2 * Ls}x{2 * nup} walkers of the same workload (numpy/LAPACK "
+                                         f"oracle)"}
         out["gpu_vs_cpu_fp64"] = out["paths"].get("fp64", out["paths"][main_path])["walker_steps_per_s"] / ws_s
     print(json.dumps(out))
 
 
 if __name__ == "__main__":
-    main()
+    sys.exit(main())
