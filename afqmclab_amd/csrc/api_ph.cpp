@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -137,6 +138,7 @@ int ph_background(afqmc_ph ph, afqmc_ph_walkers w) {
     a.B = ph->X[sp].p; a.ldb = K;
     a.C = ph->bgbuf.p; a.ldc = ph->G;
     a.beta_one = sp;
+    a.tile = GEMM_TILE_32x32;  // fastest shape for G x nw outputs with K = L Ns (both precisions)
     int rc = launch_cgemm<T>(OP_T, EPI_STORE, a, ph->ctx->stream);
     if (rc) return rc;
   }
@@ -380,6 +382,8 @@ int ph_measure_impl(afqmc_ph ph, afqmc_ph_walkers w) {
       a.B = (char*)ph->X[sp].p + ph->esz() * (size_t)w0 * ph->L * Ns; a.ldb = ph->L;
       a.C = ph->Dbuf.p; a.ldc = (long long)ph->G * Ns;
       a.n_fastest = 1;
+      // fastest shapes for (G Ns) x (Ns chunk) outputs with K = L: 128x128 in FP64, 64x64 in FP32
+      a.tile = std::is_same<T, double>::value ? GEMM_TILE_128x128 : GEMM_TILE_64x64;
       if ((rc = launch_cgemm<T>(OP_T, EPI_STORE, a, s))) return rc;
       if ((rc = launch_exchange_trace<T>(nb, ph->G, Ns, ph->Dbuf.p, ph->ex.p + (size_t)w0 * ph->G, sp, s))) return rc;
     }

@@ -443,7 +443,7 @@ int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s) {
           "cgemm: a Hermitian result needs op C, a square tile, M == N, real alpha and a plain store");
   if (epi == EPI_STORE) {
     if (opa == OP_N) return sc ? dispatch_tile<T, OP_N, EPI_STORE, true>(a, s) : dispatch_tile<T, OP_N, EPI_STORE, false>(a, s);
-    if (opa == OP_T) return dispatch_default<T, OP_T, EPI_STORE, false>(a, s);
+    if (opa == OP_T) return dispatch_tile<T, OP_T, EPI_STORE, false>(a, s);
     if (opa == OP_C) return dispatch_tile<T, OP_C, EPI_STORE, false>(a, s);
     if (opa == OP_NR) return dispatch_default<T, OP_NR, EPI_STORE, false>(a, s);
   } else {
