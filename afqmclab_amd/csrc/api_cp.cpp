@@ -169,6 +169,8 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
       a.C = cp->qrS[b].p; a.ldc = Nb; a.strideC = (long long)Nb * Nb;
       a.alive = w->alive.p;
       a.herm = 1;
+      // measured fastest for the Gram shapes: one 128x128 tile per matrix above 64 columns
+      a.tile = Nb > 64 ? GEMM_TILE_128x128 : GEMM_TILE_64x64;
       if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
       const int* mask = w->alive.p;
       if (pass == 1) {
@@ -190,6 +192,7 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
       g.C = tmp[b].p; g.ldc = Mb; g.strideC = w->wsize(b);
       g.alive = mask;
       g.b_upper = 1;  // R^{-1} is upper triangular
+      g.tile = GEMM_TILE_32x32;  // measured fastest for W R^{-1}
       if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
       if (pass == 1 &&
           (rc = launch_select_alive(w->nw, w->wsize(b), sizeof(zd), cp->qr_skip.p, wf[b].p, tmp[b].p, s)))

@@ -287,6 +287,8 @@ int ph_qr_f64(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, DevBuf<char>& t
     a.C = ph->qrS.p; a.ldc = N; a.strideC = (long long)N * N;
     a.alive = w->alive.p;
     a.herm = 1;
+    // measured fastest for the Gram shapes: one 128x128 tile per matrix above 64 columns
+    a.tile = N > 64 ? GEMM_TILE_128x128 : GEMM_TILE_64x64;
     if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
     SmallDenseArgs c;
     memset(&c, 0, sizeof(c));
@@ -301,6 +303,7 @@ int ph_qr_f64(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, DevBuf<char>& t
     g.C = tmp.p; g.ldc = M; g.strideC = w->wsize();
     g.alive = w->alive.p;
     g.b_upper = 1;  // R^{-1} is upper triangular
+    g.tile = GEMM_TILE_32x32;  // measured fastest for W R^{-1}
     if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
     std::swap(wf, tmp);
     if ((rc = launch_logw_add_real(w->nw, w->logw.p, w->alive.p, ph->qrld.p, nullptr, 0, 1.0, 1.0, s))) return rc;
