@@ -331,6 +331,85 @@ __global__ void comb_kernel(int L, const double* weight, const double* ubuf, Dev
   }
 }
 
+// The same comb with the walkers in LDS: the two running sums (the weight total and the cumulative
+// probabilities) stay sequential in one lane, so every partial sum is rounded as the reference
+// rounds it, while the teeth search the cumulative sums in parallel.  The reference's scan stops at
+// the first bin j (from the previous tooth's bin on) with !(tooth > cumulative[j]); the teeth
+// (i + u_i) / L increase with i and the cumulative sums do not decrease, so that bin is the first j
+// over the whole array where the predicate holds, found by binary search (clamped at L-1 as the scan).
+constexpr int COMB_LDS_MAX = 4096;
+__global__ __launch_bounds__(1024) void comb_lds_kernel(int L, const double* weight, const double* ubuf, DevScalars* sc,
+                                                        int* table, double* prob, int* number) {
+  __shared__ double p[COMB_LDS_MAX];
+  __shared__ int num[COMB_LDS_MAX];
+  __shared__ double s_sum;
+  const int tid = threadIdx.x, T = blockDim.x;
+  for (int i = tid; i < L; i += T) {
+    p[i] = weight[i];
+    num[i] = 0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sum = 0.0, maxW = 0.0, minW = 1e20;
+    for (int i = 0; i < L; ++i) {
+      const double w = p[i];
+      sum += w;
+      if (maxW < w) maxW = w;
+      if (minW > w) minW = w;
+    }
+    sc->wmin = minW;
+    sc->wmax = maxW;
+    sc->wavg = sum / (L * 1.0);
+    s_sum = sum;
+  }
+  __syncthreads();
+  const double sum = s_sum;
+  for (int i = tid; i < L; i += T) {
+    const double q = p[i] / sum;
+    prob[i] = q;
+    p[i] = q;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double acc = p[0];
+    for (int j = 1; j < L; ++j) {
+      acc += p[j];
+      p[j] = acc;
+    }
+  }
+  __syncthreads();
+  const unsigned long long key = sc->key;
+  const unsigned long long c0 = sc->counter;
+  for (int i = tid; i < L; i += T) {
+    const double u = afqmc_take_uniform(ubuf, sc, key, c0 + (unsigned long long)i);
+    const double tooth = (i + u) / (L * 1.0);
+    int lo = 0, hi = L - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (!(tooth > p[mid])) hi = mid;
+      else lo = mid + 1;
+    }
+    atomicAdd(&num[lo], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    sc->counter = c0 + (unsigned long long)L;
+    for (int i = 0; i < L; ++i) table[i] = (num[i] != 0) ? i : -1;
+    int zero_i = 0, one_i = 0;
+    while (zero_i < L && num[zero_i] != 0) zero_i++;
+    while (one_i < L && num[one_i] <= 1) one_i++;
+    while (zero_i < L && one_i < L) {
+      table[zero_i] = one_i;
+      num[zero_i]++;
+      num[one_i]--;
+      while (zero_i < L && num[zero_i] != 0) zero_i++;
+      while (one_i < L && num[one_i] <= 1) one_i++;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < L; i += T) number[i] = num[i];
+}
+
 template <typename T>
 __global__ void apply_table_kernel(int nw, cplx<T>* wf0, cplx<T>* wf1, long long ws0, long long ws1, zd* logw,
                                    int* alive, const int* table) {
@@ -730,7 +809,10 @@ template int launch_hs_apply_2x2<float>(int, int, int, const void*, void*, const
 
 int launch_comb(int nw, const double* weight, const double* ubuf, DevScalars* sc, int* table, double* prob,
                 int* number, hipStream_t s) {
-  hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(64), 0, s, nw, weight, ubuf, sc, table, prob, number);
+  if (nw <= COMB_LDS_MAX)
+    hipLaunchKernelGGL(comb_lds_kernel, dim3(1), dim3(1024), 0, s, nw, weight, ubuf, sc, table, prob, number);
+  else
+    hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(64), 0, s, nw, weight, ubuf, sc, table, prob, number);
   HIP_LAUNCH_CHECK();
   return 0;
 }

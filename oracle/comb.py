@@ -38,7 +38,10 @@ def popConfiguration(size, weight, u):
     s = 0.0
     for w in weight:
         s += w
-    prob = [w / s for w in weight]
+    # IEEE division as in C++: a zero total (every walker dead) gives NaN probabilities, and every
+    # tooth then stays in bin 0 (NaN comparisons are false)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        prob = [float(x) for x in np.array(weight, dtype=np.float64) / np.float64(s)]
     number = [0] * L
     index_old = 0
     prob_sum_old = prob[0]

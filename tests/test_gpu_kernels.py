@@ -152,3 +152,29 @@ def test_comb_bit_exact(L):
         ref = ocomb.popConfiguration(1, w, u)
         assert np.array_equal(t, ref)
         assert st == ocomb.popCheck(list(w))
+
+
+@pytest.mark.parametrize("case", ["equal_u0", "equal_u_half", "one_alive", "all_dead", "ties", "max_lds", "beyond_lds"])
+def test_comb_bit_exact_edges(case):
+    """Teeth on cumulative sums, a single or no nonzero weight, and walker counts at and beyond the
+    size the LDS comb holds (4096; larger counts take the sequential kernel)."""
+    from afqmclab_amd.cp import comb
+    rng = np.random.default_rng(7)
+    L = {"max_lds": 4096, "beyond_lds": 5000}.get(case, 512)
+    w = rng.uniform(0, 5, L)
+    u = rng.uniform(0, 1, L)
+    if case.startswith("equal"):
+        w[:] = 1.0
+        u[:] = 0.0 if case == "equal_u0" else 0.5
+    elif case == "one_alive":
+        w[:] = 0.0
+        w[L // 3] = 2.5
+    elif case == "all_dead":
+        w[:] = 0.0
+    elif case == "ties":
+        w = rng.integers(0, 3, L).astype(float)  # many zeros and repeated values
+        u[::2] = 0.0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ref = ocomb.popConfiguration(1, w, u)
+    t, st = comb(ctx(), w, u)
+    assert np.array_equal(t, ref)
