@@ -385,8 +385,9 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
   }
   for (int b = 0; b < cp->nblk; ++b) {
     if (cp->Nb[b] > 160) {
+      const int nb = cp->Nb[b];
       delete cp;
-      REQUIRE(false, "block column count %d exceeds the supported 160", 0);
+      REQUIRE(false, "block column count %d exceeds the supported 160", nb);
     }
   }
   cp->diag_only = (!hop && !mixing) ? 1 : 0;

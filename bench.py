@@ -355,6 +355,44 @@ def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed, tiles=None):
             "double_occupancy_bp": float(docc.real)}
 
 
+def run_hs_bandwidth(ctx, nw, steps=6):
+    """The spin-mixing HS kernel (2x2 site matrices, hs_apply_2x2_kernel) on dense 512 x 160 walkers
+    (80 + 80 electrons; dense walkers hold at most 160 columns) of the 16x16 lattice with in-plane
+    fields hx, hy: its average launch time from HIP events on the context stream against the bytes it
+    must move (read W, write W', the 4 x Ls site matrices per walker), for both precisions."""
+    from afqmclab_amd.cp import CPMethod, CPPropagator
+    from afqmclab_amd.model import HubbardModel, free_electron_trial
+    N = 160
+    base = HubbardModel.square(16, 16, N, 8.0)
+    rng = np.random.default_rng(4)
+    Ls = base.L
+    model = HubbardModel(L=Ls, N=N, K=base.K, U=np.full(Ls, 8.0), hx=rng.uniform(-0.2, 0.2, Ls),
+                         hy=rng.uniform(-0.2, 0.2, Ls))
+    phi = free_electron_trial(model)
+    out = {"workload": f"dense {2 * Ls} x {N} walkers of the 16x16 lattice with hx, hy != 0, {nw} walkers, "
+                       f"densitySpin fields, {steps} steps"}
+    for precision, name, b in ((0, "fp64", 16), (1, "mixed", 8)):
+        m = CPMethod(dt=0.005, decompType="densitySpin", walkerSizePerThread=nw)
+        prop = CPPropagator(ctx, model, m, phi, spin_blocks=False, precision=precision, max_walkers=nw)
+        ws = prop.walkers(nw)
+        ws.set_from_trial(phi)
+        prop.set_seed(11, 0)
+        prop.step(ws)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        for _ in range(steps):
+            prop.step(ws)
+        t, n = ctx.timing()["hs_apply"]
+        ctx.set_timing(False)
+        nbytes = nw * (2 * 2 * Ls * N * b + 4 * Ls * b)
+        ms = t / max(n, 1)
+        out[name] = {"ms_per_launch": ms, "bytes_per_launch": nbytes, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_GBS}
+        ws.close()
+        prop.close()
+    return out
+
+
 def cpu_worker(argv):
     """One walker stream of the CPU baseline (run as `bench.py --cpu-worker DIR RANK WALKERS SECONDS`):
     the oracle's CP step on dense 2Ls x N walkers with single-threaded BLAS, timed after set-up."""
@@ -428,6 +466,7 @@ def main():
     ap.add_argument("--ph-walkers", type=int, default=1024)
     ap.add_argument("--hetero-steps", type=int, default=200, help="heterogeneous batch leg steps (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="run the 4x4-vs-ED and 8x8 gradient legs (0 = skip)")
+    ap.add_argument("--hs-steps", type=int, default=6, help="spin-mixing HS kernel bandwidth leg steps (0 = skip)")
     ap.add_argument("--bp-measurements", type=int, default=3,
                     help="back-propagation leg: measurements with a 40-step window every 20 steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -571,6 +610,9 @@ def main():
             "speedup": graph["walker_steps_per_s"] / eager["walker_steps_per_s"],
             "same_log_overlaps": bool(same),
         }
+    if args.hs_steps > 0:
+        out["hs_apply_2x2"] = run_hs_bandwidth(ctx, args.walkers, args.hs_steps)
+        log(f"[bench] spin-mixing HS kernel: {out['hs_apply_2x2']}")
     if args.bp_measurements > 0:
         log(f"[bench] back-propagation: {args.walkers} walkers, window 40, {args.bp_measurements} measurements")
         out["back_propagation"] = run_bp_leg(ctx, model, phi, nup, args.walkers, 40, 20, args.bp_measurements,
