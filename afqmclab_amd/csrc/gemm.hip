@@ -17,8 +17,14 @@
 
 namespace {
 
+// With the Green epilogue's FP64 row dots the compiler would take more than 256 registers and
+// leave one wave per SIMD; two waves let one wave's prologue and epilogue overlap the other's
+// MFMAs (the K loop of a Green tile is only N/16 long).  Wave tiles of up to 32 x 32 fit 256
+// registers without spilling; the larger ones already use 256.
 template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
-__global__ __launch_bounds__(WM * WN * 64) void cgemm_kernel(GemmArgs p) {
+__global__ __launch_bounds__(WM * WN * 64)
+__attribute__((amdgpu_waves_per_eu((EPI == EPI_GREEN && (BM / WM / 16) * (BN / WN / 16) <= 4) ? 2 : 1)))
+void cgemm_kernel(GemmArgs p) {
   constexpr int NT = WM * WN * 64;
   constexpr int TM = BM / WM, TN = BN / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
