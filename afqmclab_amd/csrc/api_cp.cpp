@@ -174,8 +174,7 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
       if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
       const int* mask = w->alive.p;
       if (pass == 1) {
-        if ((rc = launch_ortho_check(w->nw, Nb, (const zd*)cp->qrS[b].p, w->alive.p, tol, cp->qr_need.p,
-                                     cp->qr_skip.p, s)))
+        if ((rc = launch_ortho_check(w->nw, Nb, (const zd*)cp->qrS[b].p, w->alive.p, tol, cp->qr_need.p, s)))
           return rc;
         mask = cp->qr_need.p;
       }
@@ -185,19 +184,20 @@ int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tm
       c.out = cp->qrS[b].p; c.ld_out = Nb; c.stride_out = c.stride_in;
       c.logdet = cp->qrld[b].p; c.info = cp->qrinfo[b].p; c.nsub = Nb; c.alive = mask; c.alive_div = 1;
       if ((rc = launch_chol_inverse<double>(c, s))) return rc;
+      // first pass: tmp = W R^{-1} for every walker, then the buffers swap; second pass: only the
+      // walkers that need it are copied to tmp and multiplied back into wf, the others stay put
+      if (pass == 1 && (rc = launch_select_alive(w->nw, w->wsize(b), sizeof(zd), mask, wf[b].p, tmp[b].p, s)))
+        return rc;
       GemmArgs g = gemm_base<double>();
       g.M = Mb; g.N = Nb; g.K = Nb; g.batch = w->nw;
-      g.A = wf[b].p; g.lda = Mb; g.strideA = w->wsize(b);
+      g.A = pass == 0 ? wf[b].p : tmp[b].p; g.lda = Mb; g.strideA = w->wsize(b);
       g.B = cp->qrS[b].p; g.ldb = Nb; g.strideB = (long long)Nb * Nb;
-      g.C = tmp[b].p; g.ldc = Mb; g.strideC = w->wsize(b);
+      g.C = pass == 0 ? tmp[b].p : wf[b].p; g.ldc = Mb; g.strideC = w->wsize(b);
       g.alive = mask;
       g.b_upper = 1;  // R^{-1} is upper triangular
       g.tile = GEMM_TILE_32x32;  // measured fastest for W R^{-1}
       if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
-      if (pass == 1 &&
-          (rc = launch_select_alive(w->nw, w->wsize(b), sizeof(zd), cp->qr_skip.p, wf[b].p, tmp[b].p, s)))
-        return rc;
-      std::swap(wf[b], tmp[b]);
+      if (pass == 0) std::swap(wf[b], tmp[b]);
       if ((rc = launch_logw_add_real(w->nw, w->logw.p, mask, cp->qrld[b].p, nullptr, 0, 1.0, 0.0, s))) return rc;
     }
   }
@@ -473,7 +473,6 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
   if (!rc) rc = cp->table.alloc(nw);
   if (!rc) rc = cp->comb_num.alloc(nw);
   if (!rc) rc = cp->qr_need.alloc(nw);
-  if (!rc) rc = cp->qr_skip.alloc(nw);
   if (!rc) rc = cp->comb_prob.alloc(nw);
   if (!rc) rc = cp->weight.alloc(nw);
   if (!rc) rc = cp->per_walker.alloc((size_t)nw * 8);

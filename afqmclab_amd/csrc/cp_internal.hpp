@@ -32,7 +32,7 @@ struct afqmc_cp_s {
   // workspace
   DevBuf<char> ovl[2], scale[2], d2x2, qrS[2], qr64[2], qr64b[2];
   DevBuf<zd> ldet[2], qrld[2], logO, gpart[3][2], per_walker, site_acc, site_part, den;
-  DevBuf<int> info[2], qrinfo[2], prefix, table, comb_num, qr_need, qr_skip;
+  DevBuf<int> info[2], qrinfo[2], prefix, table, comb_num, qr_need;
   DevBuf<double> weight, comb_prob, force_rec;
   DevBuf<int> aux_rec;
   int ntile[2] = {1, 1};

@@ -528,7 +528,7 @@ __global__ void site_final_kernel(EnergyArgs a) {
 }
 
 // need[w] = alive && max_ij |S_ij - delta_ij| > tol (S = Q^H Q of walker w), skip[w] = alive && !need
-__global__ void ortho_check_kernel(int N, const zd* S, const int* alive, double tol, int* need, int* skip) {
+__global__ void ortho_check_kernel(int N, const zd* S, const int* alive, double tol, int* need) {
   const int w = blockIdx.x;
   __shared__ double sh[8];
   const zd* s = S + (long long)w * N * N;
@@ -549,9 +549,7 @@ __global__ void ortho_check_kernel(int N, const zd* S, const int* alive, double 
     double t = 0.0;
     for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) t = (sh[k] > t || sh[k] != sh[k]) ? sh[k] : t;
     const int a = alive ? alive[w] : 1;
-    const int n = a && !(t <= tol);
-    need[w] = n;
-    skip[w] = a && !n;
+    need[w] = a && !(t <= tol);
   }
 }
 
@@ -842,10 +840,9 @@ int launch_energy(const EnergyArgs& a, hipStream_t s) {
   return 0;
 }
 
-int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, int* skip,
-                       hipStream_t s) {
+int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, hipStream_t s) {
   if (nw <= 0) return 0;
-  hipLaunchKernelGGL(ortho_check_kernel, dim3(nw), dim3(256), 0, s, N, S, alive, tol, need, skip);
+  hipLaunchKernelGGL(ortho_check_kernel, dim3(nw), dim3(256), 0, s, N, S, alive, tol, need);
   HIP_LAUNCH_CHECK();
   return 0;
 }

@@ -106,8 +106,7 @@ constexpr int SITE_CHUNKS = 64;
 int launch_energy(const EnergyArgs& a, hipStream_t s);
 int launch_set_et_from_energy(DevScalars* sc, hipStream_t s);
 // CholeskyQR: which walkers still need a second pass (S = Q^H Q farther than tol from I)
-int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, int* skip,
-                       hipStream_t s);
+int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, hipStream_t s);
 int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int use_imag_sub,
                          double mult0, double mult1, hipStream_t s);
 

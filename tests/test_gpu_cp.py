@@ -131,6 +131,35 @@ def test_stabilize(mode):
         assert np.max(np.abs(D - np.diag(np.diag(D)))) < 1e-10
 
 
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_stabilize_second_pass(mode):
+    """Walkers with two nearly parallel columns (condition number about 1e4) leave the first
+    CholeskyQR pass about 1e-8 from orthonormal, so they take the second pass; the
+    well-conditioned walkers beside them skip it.  Both kinds must match the oracle's QR."""
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode)
+    L = model.L
+    rng = np.random.default_rng(9)
+    wfs2 = []
+    for i, w in enumerate(wfs):
+        if i % 2:
+            w = w.copy()
+            rows = slice(0, 2 * L) if mode == "dense" else slice(0, L)
+            w[rows, 1] = w[rows, 0] + 1e-4 * crand(rng, w[rows, 0].shape[0])
+        wfs2.append(w)
+    ws.upload(wfs2, logw)
+    prop.stabilize(ws)
+    out, lw, _ = ws.download()
+    for i in range(ws.nw):
+        ref = SD(wf=wfs2[i], logw=logw[i])
+        ref.stabilize()
+        assert abs(lw[i] - ref.logw) < RTOL * max(1, abs(ref.logw)), i
+        q = out[i]
+        assert np.max(np.abs(q.conj().T @ q - np.eye(model.N))) < 1e-12, i
+        D = q.conj().T @ ref.wf
+        assert np.max(np.abs(np.abs(np.diag(D)) - 1)) < 1e-10, i
+        assert np.max(np.abs(D - np.diag(np.diag(D)))) < 1e-10, i
+
+
 @pytest.mark.parametrize("mode,decomp", [("dense", "densitySpin"), ("blocks", "densitySpin"),
                                          ("dense", "densityCharge"), ("blocks", "densityCharge"),
                                          ("dense", "hopCharge"), ("dense", "hopSpin")])
