@@ -17,10 +17,11 @@
 
 namespace {
 
-// With the Green epilogue's FP64 row dots the compiler would take more than 256 registers and
-// leave one wave per SIMD; two waves let one wave's prologue and epilogue overlap the other's
-// MFMAs (the K loop of a Green tile is only N/16 long).  Wave tiles of up to 32 x 32 fit 256
-// registers without spilling; the larger ones already use 256.
+// Green tiles with wave tiles of up to 32 x 32: left alone, the compiler gives the 64 x 64 tile
+// 216 registers (two waves per SIMD); asked for at least two waves it schedules it in 158 (three
+// waves), and more waves let one wave's prologue and epilogue overlap the others' MFMAs, which
+// matters because the K loop of a Green tile is only N/16 long.  The larger wave tiles already
+// sit at 256 registers.
 template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
 __global__ __launch_bounds__(WM * WN * 64)
 __attribute__((amdgpu_waves_per_eu((EPI == EPI_GREEN && (BM / WM / 16) * (BN / WN / 16) <= 4) ? 2 : 1)))
