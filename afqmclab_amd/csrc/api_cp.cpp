@@ -835,9 +835,11 @@ int afqmc_cp_hs_apply(afqmc_cp cp, afqmc_walkers w, const double* diag4, const d
     std::vector<zf> h(n);
     for (size_t k = 0; k < n; ++k) h[k] = cmk<float>((float)diag4[2 * k], (float)diag4[2 * k + 1]);
     HIP_TRY(hipMemcpy(d.p, h.data(), sizeof(zf) * n, hipMemcpyHostToDevice));
+    RegionTimer t(cp->ctx, REG_HS_APPLY);
     rc = launch_hs_apply_2x2<float>(w->nw, L, cp->N, w->wf[0].p, w->tmp[0].p, d.p, s);
   } else {
     HIP_TRY(hipMemcpy(d.p, diag4, sizeof(zd) * n, hipMemcpyHostToDevice));
+    RegionTimer t(cp->ctx, REG_HS_APPLY);
     rc = launch_hs_apply_2x2<double>(w->nw, L, cp->N, w->wf[0].p, w->tmp[0].p, d.p, s);
   }
   if (rc) return rc;

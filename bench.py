@@ -357,37 +357,38 @@ def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed, tiles=None):
 
 def run_hs_bandwidth(ctx, nw, steps=6):
     """The spin-mixing HS kernel (2x2 site matrices, hs_apply_2x2_kernel) on dense 512 x 160 walkers
-    (80 + 80 electrons; dense walkers hold at most 160 columns) of the 16x16 lattice with in-plane
-    fields hx, hy: its average launch time from HIP events on the context stream against the bytes it
-    must move (read W, write W', the 4 x Ls site matrices per walker), for both precisions."""
+    (80 + 80 electrons; dense walkers hold at most 160 columns) of the 16x16 lattice: `steps`
+    applications of random near-identity site matrices through afqmc_cp_hs_apply, each launch timed
+    with HIP events on the context stream, against the bytes it must move (read W, write W', the
+    4 x Ls site matrices per walker), for both precisions."""
     from afqmclab_amd.cp import CPMethod, CPPropagator
     from afqmclab_amd.model import HubbardModel, free_electron_trial
     N = 160
-    base = HubbardModel.square(16, 16, N, 8.0)
-    rng = np.random.default_rng(4)
-    Ls = base.L
-    model = HubbardModel(L=Ls, N=N, K=base.K, U=np.full(Ls, 8.0), hx=rng.uniform(-0.2, 0.2, Ls),
-                         hy=rng.uniform(-0.2, 0.2, Ls))
+    model = HubbardModel.square(16, 16, N, 8.0)
     phi = free_electron_trial(model)
-    out = {"workload": f"dense {2 * Ls} x {N} walkers of the 16x16 lattice with hx, hy != 0, {nw} walkers, "
-                       f"densitySpin fields, {steps} steps"}
+    Ls = model.L
+    rng = np.random.default_rng(4)
+    out = {"workload": f"dense {2 * Ls} x {N} walkers of the 16x16 lattice, {nw} walkers, general 2x2 site "
+                       f"matrices (spin mixing), {steps} launches"}
+    d4 = 0.1 * (rng.uniform(-1, 1, (nw, 4, Ls)) + 1j * rng.uniform(-1, 1, (nw, 4, Ls)))
+    d4[:, 0] += 1.0
+    d4[:, 3] += 1.0
+    slw = np.zeros(nw, dtype=np.complex128)
     for precision, name, b in ((0, "fp64", 16), (1, "mixed", 8)):
         m = CPMethod(dt=0.005, decompType="densitySpin", walkerSizePerThread=nw)
         prop = CPPropagator(ctx, model, m, phi, spin_blocks=False, precision=precision, max_walkers=nw)
         ws = prop.walkers(nw)
         ws.set_from_trial(phi)
-        prop.set_seed(11, 0)
-        prop.step(ws)
-        ctx.synchronize()
+        prop.hs_apply(ws, d4, slw)
         ctx.set_timing(True)
         for _ in range(steps):
-            prop.step(ws)
+            prop.hs_apply(ws, d4, slw)
         t, n = ctx.timing()["hs_apply"]
         ctx.set_timing(False)
         nbytes = nw * (2 * 2 * Ls * N * b + 4 * Ls * b)
         ms = t / max(n, 1)
-        out[name] = {"ms_per_launch": ms, "bytes_per_launch": nbytes, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
-                     "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_GBS}
+        out[name] = {"ms_per_launch": ms, "launches": n, "bytes_per_launch": nbytes,
+                     "GB_per_s": nbytes / (ms * 1e-3) / 1e9, "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_GBS}
         ws.close()
         prop.close()
     return out
@@ -466,7 +467,7 @@ def main():
     ap.add_argument("--ph-walkers", type=int, default=1024)
     ap.add_argument("--hetero-steps", type=int, default=200, help="heterogeneous batch leg steps (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="run the 4x4-vs-ED and 8x8 gradient legs (0 = skip)")
-    ap.add_argument("--hs-steps", type=int, default=6, help="spin-mixing HS kernel bandwidth leg steps (0 = skip)")
+    ap.add_argument("--hs-steps", type=int, default=6, help="launches of the spin-mixing HS kernel in its bandwidth leg (0 = skip)")
     ap.add_argument("--bp-measurements", type=int, default=3,
                     help="back-propagation leg: measurements with a 40-step window every 20 steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
