@@ -12,7 +12,9 @@
 //   m_i = x_i,k+1 / pivot (0 at the pivot row) through LDS; every wave reads the pivot row of its
 //   own columns with readlane.  A row chosen as pivot is not rescaled: it keeps the scale
 //   -pivot (the exchange step is linear in each row), which is divided out when the inverse is
-//   written.  One barrier per step.
+//   written.  One barrier per step.  For 129 to 160 rows the third row group (rows 128..159,
+//   lanes 0..31) lives in LDS, each wave holding its own columns of it: thirty complex registers
+//   per lane would not fit the 128 a 16-wave workgroup allows.
 // chol_inverse: S = R^H R (upper R, positive real diagonal) and R^{-1} in one sweep, row
 //   formulation in one array: after the Jacobi scaling S' = D^-1/2 S D^-1/2, step k leaves
 //   row k of the working array unscaled (scale r_kk: R's row in the upper part, the row of
@@ -21,7 +23,9 @@
 //   multiplier turns it into A's column (|S_kk| <= 1 after the scaling, so the cancellation
 //   costs no accuracy).  Row layout: wave w owns rows w + 16c, lane l columns l + 64r, so a row
 //   is published by one wave with coalesced LDS writes and R^{-1} = D^-1/2 A^H is written
-//   coalesced.  Returns sum log R_kk (CholeskyQR step of the walker restabilisation).
+//   coalesced.  Returns sum log R_kk (CholeskyQR step of the walker restabilisation).  For 129
+//   to 160 columns the third column group (columns 128..159, lanes 0..31) lives in LDS, as in
+//   the Gauss-Jordan kernel.
 #include "small_dense.hpp"
 
 #include <type_traits>
@@ -72,6 +76,14 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 __device__ __forceinline__ zd readlane_z(zd v, int lane) { return cmk<double>(readlane_d(v.re, lane), readlane_d(v.im, lane)); }
+// a wave-uniform value (an LDS broadcast read) moved to scalar registers
+__device__ __forceinline__ double readfirstlane_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ zd readfirstlane_z(zd v) { return cmk<double>(readfirstlane_d(v.re), readfirstlane_d(v.im)); }
 
 // one DPP step of a wave maximum: v = max(v, v of the source lane); lanes without a source (or
 // in rows outside ROW_MASK) keep their value
@@ -119,14 +131,31 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   __shared__ int qinv[NMAX];
   __shared__ int singular, parity;
 
-  zd x[NR][NC];
+  // register row groups; with NR == 3 the third group (rows 128 + lane, lane < 32) is g2[wv]
+  constexpr bool G2 = NR == 3;
+  constexpr int RR = G2 ? 2 : NR;
+  static_assert(!G2 || NMAX <= 160, "the LDS row group holds rows 128..159");
+  __shared__ zd g2[G2 ? WAVES : 1][G2 ? NC : 1][32];
+  const bool g2lane = G2 && lane < 32;
+  zd x[RR][NC];
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
+  for (int r = 0; r < RR; ++r)
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int i = lane + 64 * r, j = wv + WAVES * c;
       x[r][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
     }
+  if constexpr (G2) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int i = lane + 128, j = wv + WAVES * c;
+      if (g2lane) g2[wv][c][lane] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+    }
+  }
+  // element (lane + 128, column of register column C) of the LDS row group, updated in place
+  auto g2_fms = [&](int c, zd m2, zd rv) {
+    if (g2lane) g2[wv][c][lane] = cfms(g2[wv][c][lane], m2, rv);
+  };
   unsigned used = 0u;  // bit r: row lane + 64r has been a pivot row
   if (threadIdx.x == 0) { singular = 0; parity = 0; }
 
@@ -136,11 +165,22 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
     constexpr int C1 = decltype(c1c)::value;
     unsigned long long key = 0ull;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
+    for (int r = 0; r < RR; ++r) {
       const int i = lane + 64 * r;
       if (i < N && !((used >> r) & 1u)) {
         const unsigned long long kk = pivot_key(x[r][C1], i);
         key = kk > key ? kk : key;
+      }
+    }
+    zd v2 = cmk<double>(0, 0);
+    if constexpr (G2) {
+      if (g2lane) {
+        v2 = g2[wv][C1][lane];
+        const int i = lane + 128;
+        if (i < N && !((used >> 2) & 1u)) {
+          const unsigned long long kk = pivot_key(v2, i);
+          key = kk > key ? kk : key;
+        }
       }
     }
     key = wave_max_u64(key);
@@ -149,17 +189,24 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
     const int pr = (bi >= 0 && bi < N) ? bi : 0;
     const int rr = pr >> 6, pl = pr & 63;
     zd piv = cmk<double>(0, 0);
-    static_for<NR>([&](auto rc) {
+    static_for<RR>([&](auto rc) {
       constexpr int R = decltype(rc)::value;
       if (rr == R) piv = readlane_z(x[R][C1], pl);
     });
+    if constexpr (G2) {
+      if (rr == 2) piv = readfirstlane_z(g2[wv][C1][pl]);
+    }
     const bool nz = abs1(piv) > 0.0;
     const double id = nz ? recip(piv.re * piv.re + piv.im * piv.im) : 0.0;
     const zd inv = cmk<double>(piv.re * id, -piv.im * id);
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
+    for (int r = 0; r < RR; ++r) {
       const int i = lane + 64 * r;
       colbuf[buf][i] = (i == pr) ? cmk<double>(0, 0) : cmul(x[r][C1], inv);
+    }
+    if constexpr (G2) {
+      const int i = lane + 128;
+      colbuf[buf][i] = (i == pr) ? cmk<double>(0, 0) : cmul(v2, inv);
     }
     if (lane == 0) {
       pivrow[buf] = pr;
@@ -187,25 +234,33 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
     if (lane == pl) used |= 1u << rr;
     // the pivot row of this wave's columns (uniform)
     zd rv[NC];
-    static_for<NR>([&](auto rc) {
+    static_for<RR>([&](auto rc) {
       constexpr int R = decltype(rc)::value;
       if (rr == R) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) rv[c] = readlane_z(x[R][c], pl);
       }
     });
+    if constexpr (G2) {
+      if (rr == 2) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) rv[c] = readfirstlane_z(g2[wv][c][pl]);
+      }
+    }
     const bool next_here = (k + 1 < N) && (wv == (t + 1) % WAVES);
     const bool next_in_cb = (t + 1 < WAVES);
     // look-ahead: column k+1 first, then its pivot and multipliers
     if (next_here) {
       if (next_in_cb) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) x[r][CB] = cfms(x[r][CB], m[r], rv[CB]);
+        for (int r = 0; r < RR; ++r) x[r][CB] = cfms(x[r][CB], m[r], rv[CB]);
+        if constexpr (G2) g2_fms(CB, m[2], rv[CB]);
         publish(std::integral_constant<int, CB>{}, k + 1, nbuf);
       } else {
         if constexpr (CB + 1 < NC) {
 #pragma unroll
-          for (int r = 0; r < NR; ++r) x[r][CB + 1] = cfms(x[r][CB + 1], m[r], rv[CB + 1]);
+          for (int r = 0; r < RR; ++r) x[r][CB + 1] = cfms(x[r][CB + 1], m[r], rv[CB + 1]);
+          if constexpr (G2) g2_fms(CB + 1, m[2], rv[CB + 1]);
           publish(std::integral_constant<int, CB + 1>{}, k + 1, nbuf);
         }
       }
@@ -216,12 +271,16 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
       const bool done = next_here && ((next_in_cb && c == CB) || (!next_in_cb && c == CB + 1));
       if (done || (owner && c == CB)) continue;
 #pragma unroll
-      for (int r = 0; r < NR; ++r) x[r][c] = cfms(x[r][c], m[r], rv[c]);
+      for (int r = 0; r < RR; ++r) x[r][c] = cfms(x[r][c], m[r], rv[c]);
+      if constexpr (G2) g2_fms(c, m[2], rv[c]);
     }
     // column k becomes the multipliers, -1 at the (unscaled) pivot row
     if (owner) {
 #pragma unroll
-      for (int r = 0; r < NR; ++r) x[r][CB] = (lane + 64 * r == pr) ? cmk<double>(-1.0, 0.0) : m[r];
+      for (int r = 0; r < RR; ++r) x[r][CB] = (lane + 64 * r == pr) ? cmk<double>(-1.0, 0.0) : m[r];
+      if constexpr (G2) {
+        if (g2lane) g2[wv][CB][lane] = (lane + 128 == pr) ? cmk<double>(-1.0, 0.0) : m[2];
+      }
     }
     __syncthreads();
   };
@@ -279,7 +338,7 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   if (!p.write_inverse) return;
   // inverse: O^{-1}[qinv[i]][perm[j]] = x[i][j] / (-pivot of row i)
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
+  for (int r = 0; r < RR; ++r) {
     const int i = lane + 64 * r;
     if (i >= N) continue;
     const int row = qinv[i];
@@ -289,6 +348,19 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
       const int j = wv + WAVES * c;
       if (j >= N) continue;
       out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(x[r][c], sc));
+    }
+  }
+  if constexpr (G2) {
+    const int i = lane + 128;
+    if (g2lane && i < N) {
+      const int row = qinv[i];
+      const zd sc = rowscale[i];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int j = wv + WAVES * c;
+        if (j >= N) continue;
+        out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(g2[wv][c][lane], sc));
+      }
     }
   }
 }
@@ -314,14 +386,27 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
 
   // S is Hermitian: element (i, j) is read as conj(S(j, i)), so consecutive lanes read
   // consecutive addresses
-  zd y[NC][NR];
+  // register column groups; with NR == 3 the third group (columns 128 + lane, lane < 32) is g2[wv]
+  constexpr bool G2 = NR == 3;
+  constexpr int RR = G2 ? 2 : NR;
+  static_assert(!G2 || NMAX <= 160, "the LDS column group holds columns 128..159");
+  __shared__ zd g2[G2 ? WAVES : 1][G2 ? NC : 1][32];
+  const bool g2lane = G2 && lane < 32;
+  zd y[NC][RR];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
+    for (int r = 0; r < RR; ++r) {
       const int i = wv + WAVES * c, j = lane + 64 * r;
       y[c][r] = (i < N && j < N) ? cconj(to_zd(in[j + (long long)i * p.ld_in])) : cmk<double>(0, 0);
     }
+  if constexpr (G2) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int i = wv + WAVES * c, j = lane + 128;
+      if (g2lane) g2[wv][c][lane] = (i < N && j < N) ? cconj(to_zd(in[j + (long long)i * p.ld_in])) : cmk<double>(0, 0);
+    }
+  }
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
 #pragma unroll
@@ -330,7 +415,9 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
     for (int r = 0; r < NR; ++r) {
       const int i = wv + WAVES * c, j = lane + 64 * r;
       if (i == j && i < N) {
-        double d = y[c][r].re;
+        double d;
+        if constexpr (G2) d = r < RR ? y[c][r < RR ? r : 0].re : g2[wv][c][lane].re;
+        else d = y[c][r].re;
         if (!(d > 0.0)) { bad = 1; d = 1.0; }
         dsq[i] = 1.0 / sqrt(d);
         logr[i] = 0.5 * log(d);
@@ -349,7 +436,10 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
     if (i >= N) continue;
     const double di = dsq[i];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) y[c][r] = cscale(y[c][r], di * dcol[r]);
+    for (int r = 0; r < RR; ++r) y[c][r] = cscale(y[c][r], di * dcol[r]);
+    if constexpr (G2) {
+      if (g2lane) g2[wv][c][lane] = cscale(g2[wv][c][lane], di * dcol[2]);
+    }
   }
 
   // The owner of row k (register row CK of this wave, updated through step k-1) publishes it.
@@ -357,16 +447,23 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
     constexpr int CK = decltype(ckc)::value;
     const int kr = k >> 6, kl = k & 63;
     double skk = 1.0;
-    static_for<NR>([&](auto rc) {
+    static_for<RR>([&](auto rc) {
       constexpr int R = decltype(rc)::value;
       if (kr == R) skk = readlane_d(y[CK][R].re, kl);
     });
+    if constexpr (G2) {
+      if (kr == 2) skk = readfirstlane_d(g2[wv][CK][kl].re);
+    }
     const bool ok = skk > 0.0;
     if (!ok) skk = 1.0;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
+    for (int r = 0; r < RR; ++r) {
       const int j = lane + 64 * r;
       rowbuf[buf][j] = (j == k) ? cmk<double>(skk + 1.0, 0.0) : y[CK][r];
+    }
+    if constexpr (G2) {
+      const int j = lane + 128;
+      rowbuf[buf][j] = (j == k) ? cmk<double>(skk + 1.0, 0.0) : (g2lane ? g2[wv][CK][lane] : cmk<double>(0, 0));
     }
     if (lane == 0) {
       iskkbuf[buf] = recip(skk);
@@ -395,7 +492,10 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
       const int i = wv + WAVES * C;
       const zd mi = cscale(cconj(rowbuf[buf][i]), iskk);
 #pragma unroll
-      for (int r = 0; r < NR; ++r) y[C][r] = cfms(y[C][r], mi, rk[r]);
+      for (int r = 0; r < RR; ++r) y[C][r] = cfms(y[C][r], mi, rk[r]);
+      if constexpr (G2) {
+        if (g2lane) g2[wv][C][lane] = cfms(g2[wv][C][lane], mi, rk[2]);
+      }
     };
     if (next_here) {
       if (next_in_cb) {
@@ -455,8 +555,11 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
     for (int r = 0; r < NR; ++r) {
       const int j = lane + 64 * r;
       if (j >= N) continue;
+      zd yv;
+      if constexpr (G2) yv = r < RR ? y[c][r < RR ? r : 0] : g2[wv][c][lane];
+      else yv = y[c][r];
       zd v;
-      if (j < i) v = cscale(cconj(y[c][r]), ri * dcol[r]);
+      if (j < i) v = cscale(cconj(yv), ri * dcol[r]);
       else if (j == i) v = cmk<double>(ri * dcol[r], 0.0);
       else v = cmk<double>(0, 0);
       out[j + (long long)i * p.ld_out] = from_zd<T>(v);

@@ -97,7 +97,7 @@ def test_zgemm_batched_strided_and_scaled(precision):
             assert np.max(np.abs(C[b][:, cols] - ref)) < TOL[precision] * 30
 
 
-@pytest.mark.parametrize("N", [1, 3, 10, 16, 17, 40, 64, 65, 112, 128])
+@pytest.mark.parametrize("N", [1, 3, 10, 16, 17, 40, 64, 65, 112, 128, 129, 150, 160])
 def test_gj_inverse_and_logdet(N):
     rng = np.random.default_rng(N)
     batch = 6
@@ -112,6 +112,22 @@ def test_gj_inverse_and_logdet(N):
         assert -np.pi < ld[b].imag <= np.pi
 
 
+@pytest.mark.parametrize("N", [112, 150])
+def test_gj_inverse_fp32_storage(N):
+    """FP32 input and output, FP64 elimination (the mixed path): the inverse to FP32 rounding and the
+    log-determinant of the FP32-rounded matrix to FP64 accuracy."""
+    rng = np.random.default_rng(7 + N)
+    batch = 4
+    A = (crand(rng, batch, N, N) + 0.5 * np.eye(N)).astype(np.complex64).astype(np.complex128)
+    inv, ld, info = gj_inverse(A, precision=1)
+    for b in range(batch):
+        assert info[b] == 0
+        assert np.max(np.abs(inv[b] @ A[b] - np.eye(N))) < 1e-4
+        ref = linalg.log_determinant(*linalg.lu_construct(A[b]))
+        assert abs(ld[b].real - ref.real) < 1e-11 * max(1, abs(ref.real))
+        assert abs(np.exp(1j * (ld[b].imag - ref.imag)) - 1) < 1e-11
+
+
 def test_gj_pivoting_cases():
     # permutation matrix (one swap -> det -1), exactly singular matrix, identity
     P = np.eye(8, dtype=complex)[[1, 0, 2, 3, 4, 5, 6, 7]]
@@ -123,7 +139,7 @@ def test_gj_pivoting_cases():
     assert np.allclose(inv[2], np.eye(8)) and abs(ld[2]) == 0
 
 
-@pytest.mark.parametrize("N", [5, 28, 64, 112])
+@pytest.mark.parametrize("N", [5, 28, 64, 112, 129, 160])
 def test_chol_inverse(N):
     rng = np.random.default_rng(100 + N)
     batch = 4
