@@ -110,10 +110,12 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
   return ((unsigned long long)hi << 32) | lo;
 }
 
-// Gauss-Jordan, column layout: x[r][c] = element (lane + 64r, wave + 16c); NR * 64 >= N and
-// NC * 16 >= N.
-template <typename T, int NR, int NC>
-__global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
+// Gauss-Jordan, column layout: x[r][c] = element (lane + 64r, wave + WV c); NR * 64 >= N and
+// NC * WV >= N.  The last row group keeps its first NL register columns in registers and the
+// others in LDS (gl[wave][c - NL][lane], lanes below LL): all of them for 129..160 rows, the
+// second half for the 8-wave kernel of 97..112 rows, none otherwise.
+template <typename T, int NR, int NC, int WV, int NL>
+__global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(4))) void gj_kernel(SmallDenseArgs p) {
   const int bidx = blockIdx.x;
   if (p.alive && !p.alive[bidx / p.alive_div]) return;
   const int N = p.N;
@@ -131,30 +133,38 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   __shared__ int qinv[NMAX];
   __shared__ int singular, parity;
 
-  // register row groups; with NR == 3 the third group (rows 128 + lane, lane < 32) is g2[wv]
-  constexpr bool G2 = NR == 3;
-  constexpr int RR = G2 ? 2 : NR;
-  static_assert(!G2 || NMAX <= 160, "the LDS row group holds rows 128..159");
-  __shared__ zd g2[G2 ? WAVES : 1][G2 ? NC : 1][32];
-  const bool g2lane = G2 && lane < 32;
-  zd x[RR][NC];
+  constexpr int RL = NR - 1;                   // the last row group
+  constexpr bool LG = NL < NC;                 // part of it in LDS
+  constexpr int LL = 64 * NR <= NMAX ? 64 : NMAX - 64 * RL;  // its lanes that can hold a row
+  static_assert(LL > 0 && LL <= 64, "row groups");
+  __shared__ zd gl[LG ? WV : 1][LG ? NC - NL : 1][LG ? LL : 1];
+  const bool llane = lane < LL;
+  // whether element (r, c) is a register (folded at compile time once the loops are unrolled)
+  auto reg = [](int r, int c) { return r < RL || c < NL; };
+  auto lds = [&](int c, int l) -> zd& { return gl[wv][LG && c >= NL ? c - NL : 0][LG ? l : 0]; };
+  zd x[NR][NC];
 #pragma unroll
-  for (int r = 0; r < RR; ++r)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int i = lane + 64 * r, j = wv + WAVES * c;
-      x[r][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
-    }
-  if constexpr (G2) {
+  for (int r = 0; r < NR; ++r)
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const int i = lane + 128, j = wv + WAVES * c;
-      if (g2lane) g2[wv][c][lane] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+      const int i = lane + 64 * r, j = wv + WV * c;
+      const zd v = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+      if (reg(r, c)) x[r][c] = v;
+      else if (llane) lds(c, lane) = v;
     }
-  }
-  // element (lane + 128, column of register column C) of the LDS row group, updated in place
-  auto g2_fms = [&](int c, zd m2, zd rv) {
-    if (g2lane) g2[wv][c][lane] = cfms(g2[wv][c][lane], m2, rv);
+  // element (r, c) -= m * rv
+  auto fms = [&](int r, int c, zd m, zd rv) {
+    if (reg(r, c)) x[r][c] = cfms(x[r][c], m, rv);
+    else if (llane) lds(c, lane) = cfms(lds(c, lane), m, rv);
+  };
+  auto get = [&](int r, int c) -> zd {
+    if (reg(r, c)) return x[r][c];
+    return llane ? lds(c, lane) : cmk<double>(0, 0);
+  };
+  // element (row of lane pl in group r, c), wave-uniform, in scalar registers
+  auto row_value = [&](int r, int c, int pl) -> zd {
+    if (reg(r, c)) return readlane_z(x[r][c], pl);
+    return readfirstlane_z(lds(c, pl));
   };
   unsigned used = 0u;  // bit r: row lane + 64r has been a pivot row
   if (threadIdx.x == 0) { singular = 0; parity = 0; }
@@ -164,23 +174,14 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   auto publish = [&](auto c1c, int col, int buf) {
     constexpr int C1 = decltype(c1c)::value;
     unsigned long long key = 0ull;
+    zd v[NR];
 #pragma unroll
-    for (int r = 0; r < RR; ++r) {
+    for (int r = 0; r < NR; ++r) {
+      v[r] = get(r, C1);
       const int i = lane + 64 * r;
       if (i < N && !((used >> r) & 1u)) {
-        const unsigned long long kk = pivot_key(x[r][C1], i);
+        const unsigned long long kk = pivot_key(v[r], i);
         key = kk > key ? kk : key;
-      }
-    }
-    zd v2 = cmk<double>(0, 0);
-    if constexpr (G2) {
-      if (g2lane) {
-        v2 = g2[wv][C1][lane];
-        const int i = lane + 128;
-        if (i < N && !((used >> 2) & 1u)) {
-          const unsigned long long kk = pivot_key(v2, i);
-          key = kk > key ? kk : key;
-        }
       }
     }
     key = wave_max_u64(key);
@@ -189,24 +190,17 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
     const int pr = (bi >= 0 && bi < N) ? bi : 0;
     const int rr = pr >> 6, pl = pr & 63;
     zd piv = cmk<double>(0, 0);
-    static_for<RR>([&](auto rc) {
+    static_for<NR>([&](auto rc) {
       constexpr int R = decltype(rc)::value;
-      if (rr == R) piv = readlane_z(x[R][C1], pl);
+      if (rr == R) piv = row_value(R, C1, pl);
     });
-    if constexpr (G2) {
-      if (rr == 2) piv = readfirstlane_z(g2[wv][C1][pl]);
-    }
     const bool nz = abs1(piv) > 0.0;
     const double id = nz ? recip(piv.re * piv.re + piv.im * piv.im) : 0.0;
     const zd inv = cmk<double>(piv.re * id, -piv.im * id);
 #pragma unroll
-    for (int r = 0; r < RR; ++r) {
+    for (int r = 0; r < NR; ++r) {
       const int i = lane + 64 * r;
-      colbuf[buf][i] = (i == pr) ? cmk<double>(0, 0) : cmul(x[r][C1], inv);
-    }
-    if constexpr (G2) {
-      const int i = lane + 128;
-      colbuf[buf][i] = (i == pr) ? cmk<double>(0, 0) : cmul(v2, inv);
+      colbuf[buf][i] = (i == pr) ? cmk<double>(0, 0) : cmul(v[r], inv);
     }
     if (lane == 0) {
       pivrow[buf] = pr;
@@ -220,11 +214,11 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   if (wv == 0) publish(std::integral_constant<int, 0>{}, 0, 0);
   __syncthreads();
 
-  // Step k = CB*16 + t: column k is register column CB of wave t, column k+1 register column CB
+  // Step k = CB*WV + t: column k is register column CB of wave t, column k+1 register column CB
   // of wave t+1 (or CB+1 of wave 0), so no register array is indexed by a run-time value.
   auto step = [&](auto cbc, int t) {
     constexpr int CB = decltype(cbc)::value;
-    const int k = CB * WAVES + t;
+    const int k = CB * WV + t;
     const int buf = k & 1, nbuf = buf ^ 1;
     const int pr = __builtin_amdgcn_readfirstlane(pivrow[buf]);
     const int rr = pr >> 6, pl = pr & 63;
@@ -234,33 +228,25 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
     if (lane == pl) used |= 1u << rr;
     // the pivot row of this wave's columns (uniform)
     zd rv[NC];
-    static_for<RR>([&](auto rc) {
+    static_for<NR>([&](auto rc) {
       constexpr int R = decltype(rc)::value;
       if (rr == R) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) rv[c] = readlane_z(x[R][c], pl);
+        for (int c = 0; c < NC; ++c) rv[c] = row_value(R, c, pl);
       }
     });
-    if constexpr (G2) {
-      if (rr == 2) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) rv[c] = readfirstlane_z(g2[wv][c][pl]);
-      }
-    }
-    const bool next_here = (k + 1 < N) && (wv == (t + 1) % WAVES);
-    const bool next_in_cb = (t + 1 < WAVES);
+    const bool next_here = (k + 1 < N) && (wv == (t + 1) % WV);
+    const bool next_in_cb = (t + 1 < WV);
     // look-ahead: column k+1 first, then its pivot and multipliers
     if (next_here) {
       if (next_in_cb) {
 #pragma unroll
-        for (int r = 0; r < RR; ++r) x[r][CB] = cfms(x[r][CB], m[r], rv[CB]);
-        if constexpr (G2) g2_fms(CB, m[2], rv[CB]);
+        for (int r = 0; r < NR; ++r) fms(r, CB, m[r], rv[CB]);
         publish(std::integral_constant<int, CB>{}, k + 1, nbuf);
       } else {
         if constexpr (CB + 1 < NC) {
 #pragma unroll
-          for (int r = 0; r < RR; ++r) x[r][CB + 1] = cfms(x[r][CB + 1], m[r], rv[CB + 1]);
-          if constexpr (G2) g2_fms(CB + 1, m[2], rv[CB + 1]);
+          for (int r = 0; r < NR; ++r) fms(r, CB + 1, m[r], rv[CB + 1]);
           publish(std::integral_constant<int, CB + 1>{}, k + 1, nbuf);
         }
       }
@@ -271,22 +257,22 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
       const bool done = next_here && ((next_in_cb && c == CB) || (!next_in_cb && c == CB + 1));
       if (done || (owner && c == CB)) continue;
 #pragma unroll
-      for (int r = 0; r < RR; ++r) x[r][c] = cfms(x[r][c], m[r], rv[c]);
-      if constexpr (G2) g2_fms(c, m[2], rv[c]);
+      for (int r = 0; r < NR; ++r) fms(r, c, m[r], rv[c]);
     }
     // column k becomes the multipliers, -1 at the (unscaled) pivot row
     if (owner) {
 #pragma unroll
-      for (int r = 0; r < RR; ++r) x[r][CB] = (lane + 64 * r == pr) ? cmk<double>(-1.0, 0.0) : m[r];
-      if constexpr (G2) {
-        if (g2lane) g2[wv][CB][lane] = (lane + 128 == pr) ? cmk<double>(-1.0, 0.0) : m[2];
+      for (int r = 0; r < NR; ++r) {
+        const zd v = (lane + 64 * r == pr) ? cmk<double>(-1.0, 0.0) : m[r];
+        if (reg(r, CB)) x[r][CB] = v;
+        else if (llane) lds(CB, lane) = v;
       }
     }
     __syncthreads();
   };
   auto block = [&](auto cbc) {
     constexpr int CB = decltype(cbc)::value;
-    for (int t = 0; t < WAVES && CB * WAVES + t < N; ++t) step(cbc, t);
+    for (int t = 0; t < WV && CB * WV + t < N; ++t) step(cbc, t);
   };
   static_for<NC>(block);
 
@@ -294,7 +280,7 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   // sign of the row permutation: parity of its inversions, counted in parallel
   {
     int odd = 0;
-    for (int e = threadIdx.x; e < N * N; e += 64 * WAVES) {
+    for (int e = threadIdx.x; e < N * N; e += 64 * WV) {
       const int i = e / N, j = e - i * N;
       odd ^= (i < j && perm[i] > perm[j]) ? 1 : 0;
     }
@@ -338,29 +324,16 @@ __global__ __launch_bounds__(64 * WAVES) void gj_kernel(SmallDenseArgs p) {
   if (!p.write_inverse) return;
   // inverse: O^{-1}[qinv[i]][perm[j]] = x[i][j] / (-pivot of row i)
 #pragma unroll
-  for (int r = 0; r < RR; ++r) {
+  for (int r = 0; r < NR; ++r) {
     const int i = lane + 64 * r;
     if (i >= N) continue;
     const int row = qinv[i];
     const zd sc = rowscale[i];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const int j = wv + WAVES * c;
+      const int j = wv + WV * c;
       if (j >= N) continue;
-      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(x[r][c], sc));
-    }
-  }
-  if constexpr (G2) {
-    const int i = lane + 128;
-    if (g2lane && i < N) {
-      const int row = qinv[i];
-      const zd sc = rowscale[i];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int j = wv + WAVES * c;
-        if (j >= N) continue;
-        out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(g2[wv][c][lane], sc));
-      }
+      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(get(r, c), sc));
     }
   }
 }
@@ -567,11 +540,28 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
   }
 }
 
+// compute units of the current device (cached per device)
+int cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+
+template <typename T, int NR, int NC, int WV = WAVES, int NL = NC>
+int run_gj(const SmallDenseArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((gj_kernel<T, NR, NC, WV, NL>), dim3(a.batch), dim3(64 * WV), 0, s, a);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
 template <typename T, int NR, int NC>
-int run(int kind, const SmallDenseArgs& a, hipStream_t s) {
-  dim3 grid(a.batch), block(64 * WAVES);
-  if (kind == 0) hipLaunchKernelGGL((gj_kernel<T, NR, NC>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((chol_kernel<T, NR, NC>), grid, block, 0, s, a);
+int run_chol(const SmallDenseArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((chol_kernel<T, NR, NC>), dim3(a.batch), dim3(64 * WAVES), 0, s, a);
   HIP_LAUNCH_CHECK();
   return 0;
 }
@@ -581,13 +571,25 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
   REQUIRE(a.N >= 1 && a.N <= NMAX, "small dense kernels support 1 <= N <= %d (got %d)", NMAX, a.N);
   REQUIRE(a.alive_div >= 1, "alive_div must be >= 1");
   if (a.batch <= 0) return 0;
-  if (a.N <= 16) return run<T, 1, 1>(kind, a, s);
-  if (a.N <= 32) return run<T, 1, 2>(kind, a, s);
-  if (a.N <= 64) return run<T, 1, 4>(kind, a, s);
-  if (a.N <= 96) return run<T, 2, 6>(kind, a, s);
-  if (a.N <= 112) return run<T, 2, 7>(kind, a, s);
-  if (a.N <= 128) return run<T, 2, 8>(kind, a, s);
-  return run<T, 3, 10>(kind, a, s);
+  if (kind == 0) {
+    if (a.N <= 16) return run_gj<T, 1, 1>(a, s);
+    if (a.N <= 32) return run_gj<T, 1, 2>(a, s);
+    if (a.N <= 64) return run_gj<T, 1, 4>(a, s);
+    if (a.N <= 96) return run_gj<T, 2, 6>(a, s);
+    // 8 waves per matrix with half of the second row group in LDS: two matrices share a CU and
+    // each hides the other's per-step synchronisation (2048 x 112^2: 1.14 against 1.36 ms), which
+    // pays once there are more matrices than CUs
+    if (a.N <= 112) return a.batch > cu_count() ? run_gj<T, 2, 14, 8, 7>(a, s) : run_gj<T, 2, 7>(a, s);
+    if (a.N <= 128) return run_gj<T, 2, 8>(a, s);
+    return run_gj<T, 3, 10, WAVES, 0>(a, s);
+  }
+  if (a.N <= 16) return run_chol<T, 1, 1>(a, s);
+  if (a.N <= 32) return run_chol<T, 1, 2>(a, s);
+  if (a.N <= 64) return run_chol<T, 1, 4>(a, s);
+  if (a.N <= 96) return run_chol<T, 2, 6>(a, s);
+  if (a.N <= 112) return run_chol<T, 2, 7>(a, s);
+  if (a.N <= 128) return run_chol<T, 2, 8>(a, s);
+  return run_chol<T, 3, 10>(a, s);
 }
 
 }  // namespace

@@ -128,6 +128,30 @@ def test_gj_inverse_fp32_storage(N):
         assert abs(np.exp(1j * (ld[b].imag - ref.imag)) - 1) < 1e-11
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("N", [100, 112])
+def test_gj_inverse_large_batch(N, precision):
+    """More matrices than CUs take the 8-wave kernel with part of the matrix in LDS: the same bar
+    as the single-workgroup kernel, and FP32 outputs identical to it bit for bit."""
+    rng = np.random.default_rng(30 + N + precision)
+    batch = 600
+    A = crand(rng, batch, N, N) + 0.5 * np.eye(N)
+    if precision:
+        A = A.astype(np.complex64).astype(np.complex128)
+    inv, ld, info = gj_inverse(A, precision=precision)
+    assert not info.any()
+    tol = 1e-4 if precision else 1e-11 * N
+    for b in range(0, batch, 23):
+        assert np.max(np.abs(inv[b] @ A[b] - np.eye(N))) < tol, b
+        ref = linalg.log_determinant(*linalg.lu_construct(A[b]))
+        assert abs(ld[b].real - ref.real) < 1e-11 * max(1, abs(ref.real)), b
+        assert abs(np.exp(1j * (ld[b].imag - ref.imag)) - 1) < 1e-11, b
+    few = gj_inverse(A[:5], precision=precision)
+    if precision:
+        assert np.array_equal(few[0], inv[:5])
+    assert np.max(np.abs(few[1] - ld[:5])) < 1e-12
+
+
 def test_gj_pivoting_cases():
     # permutation matrix (one swap -> det -1), exactly singular matrix, identity
     P = np.eye(8, dtype=complex)[[1, 0, 2, 3, 4, 5, 6, 7]]
