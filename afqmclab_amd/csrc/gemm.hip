@@ -95,6 +95,29 @@ void cgemm_kernel(GemmArgs p) {
   bool ina[A_ELEMS], inb[B_ELEMS];
   const bool kpos = p.K > 0;
 
+  // Load addressing is fixed over the K loop: a thread's element e sits at a per-thread base plus
+  // e times a uniform stride, plus the K offset (k0 * lda for A in op N, k0 otherwise), so a tile
+  // costs one 64-bit add per element instead of the index arithmetic.
+  constexpr bool A_NMAJ = OPA == OP_N || OPA == OP_NR;
+  static_assert(NT % BM == 0 && NT % BK == 0, "load layout");
+  constexpr int AE = A_NMAJ ? NT / BM : NT / BK;  // rows of k (op N) or m (op T/C) per element step
+  const int a_m = A_NMAJ ? tid % BM : tid / BK;  // first element's m (+ e AE for op T/C)
+  const int a_k = A_NMAJ ? tid / BM : tid % BK;  // first element's k (+ e AE for op N)
+  const long long a_base = A_NMAJ ? (long long)(m0 + a_m) + (long long)a_k * p.lda
+                                  : (long long)a_k + (long long)(m0 + a_m) * p.lda;
+  const long long a_estride = (long long)AE * p.lda;
+  const long long a_kstride = A_NMAJ ? p.lda : 1;
+  constexpr int BE = NT / BK;
+  const int b_k = tid % BK, b_n = tid / BK;
+  const long long b_base = (long long)b_k + (long long)(n0 + b_n) * p.ldb;
+  const long long b_estride = (long long)BE * p.ldb;
+  long long s_base[SCALE ? B_ELEMS : 1];
+  if (SCALE) {
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e)
+      s_base[e] = (long long)((n0 + b_n + e * BE) / p.scale_group) * p.scale_strideG + b_k;
+  }
+
   auto load_tiles = [&](int k0) {
     if (!kpos) {
 #pragma unroll
@@ -103,30 +126,23 @@ void cgemm_kernel(GemmArgs p) {
       for (int e = 0; e < B_ELEMS; ++e) inb[e] = false;
       return;
     }
+    const long long a_k0 = (long long)k0 * a_kstride;
 #pragma unroll
     for (int e = 0; e < A_ELEMS; ++e) {
-      int idx = tid + e * NT;
-      int m, k;
-      if (OPA == OP_N || OPA == OP_NR) { m = idx % BM; k = idx / BM; }
-      else { k = idx % BK; m = idx / BK; }
-      int gm = m0 + m, gk = k0 + k;
+      const int gm = m0 + a_m + (A_NMAJ ? 0 : e * AE), gk = k0 + a_k + (A_NMAJ ? e * AE : 0);
       const bool in = gm < p.M && gk < p.K;
       ina[e] = in;
-      const long long off = in ? ((OPA == OP_N || OPA == OP_NR) ? gm + (long long)gk * p.lda
-                                                                : gk + (long long)gm * p.lda)
-                               : 0;
+      const long long off = in ? a_base + e * a_estride + a_k0 : 0;
       if (OPA == OP_NR) ra[e] = cmk<T>(Ar[off], (T)0);
       else ra[e] = A[off];
     }
+    const bool bk_in = k0 + b_k < p.K;
 #pragma unroll
     for (int e = 0; e < B_ELEMS; ++e) {
-      int idx = tid + e * NT;
-      int k = idx % BK, n = idx / BK;
-      int gk = k0 + k, gn = n0 + n;
-      const bool in = gk < p.K && gn < p.N;
+      const bool in = bk_in && n0 + b_n + e * BE < p.N;
       inb[e] = in;
-      rb[e] = Bm[in ? gk + (long long)gn * p.ldb : 0];
-      if (SCALE) rs[e] = S[in ? (long long)(gn / p.scale_group) * p.scale_strideG + gk : 0];
+      rb[e] = Bm[in ? b_base + e * b_estride + k0 : 0];
+      if (SCALE) rs[e] = S[in ? s_base[e] + k0 : 0];
     }
   };
   auto store_tiles = [&]() {
