@@ -1,5 +1,6 @@
-// Per-walker N x N dense kernels: one workgroup of 16 waves per matrix, the matrix register
-// resident (a 112 x 112 complex FP64 matrix is 196 KB, more than a CU's LDS).
+// Per-walker N x N dense kernels: one workgroup per matrix, the matrix register resident (a
+// 112 x 112 complex FP64 matrix is 196 KB, more than a CU's LDS), 16 waves, or 8 waves with part
+// of the matrix in LDS so that two matrices share a CU.
 //
 // gj_inverse_logdet: Gauss-Jordan exchange sweep with partial pivoting.  The pivot of column k
 //   is the largest |re|+|im| among rows not yet used, first index on ties, as izamax in zgetrf,
@@ -14,7 +15,10 @@
 //   -pivot (the exchange step is linear in each row), which is divided out when the inverse is
 //   written.  One barrier per step.  For 129 to 160 rows the third row group (rows 128..159,
 //   lanes 0..31) lives in LDS, each wave holding its own columns of it: thirty complex registers
-//   per lane would not fit the 128 a 16-wave workgroup allows.
+//   per lane would not fit the 128 a 16-wave workgroup allows.  For 97 to 112 rows and more
+//   matrices than CUs, 8 waves own 14 columns each and the last 7 of them keep their second row
+//   group in LDS: 128 registers and 68 KB of LDS per matrix let two matrices share a CU, and
+//   each hides the other's per-step synchronisation.
 // chol_inverse: S = R^H R (upper R, positive real diagonal) and R^{-1} in one sweep, row
 //   formulation in one array: after the Jacobi scaling S' = D^-1/2 S D^-1/2, step k leaves
 //   row k of the working array unscaled (scale r_kk: R's row in the upper part, the row of
