@@ -136,12 +136,20 @@ def test_gj_inverse_large_batch(N, precision):
     rng = np.random.default_rng(30 + N + precision)
     batch = 600
     A = crand(rng, batch, N, N) + 0.5 * np.eye(N)
+    # an exactly singular matrix (a zero row in the second row group) and a permutation matrix
+    # with pivots in both row groups
+    A[7][N - 1] = 0
+    A[11] = np.eye(N)[np.r_[64:N, 0:64]]
     if precision:
         A = A.astype(np.complex64).astype(np.complex128)
     inv, ld, info = gj_inverse(A, precision=precision)
-    assert not info.any()
+    assert info[7] == 1 and ld[7].real == -1e300
+    assert np.count_nonzero(info) == 1
+    assert np.array_equal(inv[11], A[11].T)
+    perm_sign = np.linalg.det(A[11]).real
+    assert abs(np.exp(ld[11]) - perm_sign) < 1e-12
     tol = 1e-4 if precision else 1e-11 * N
-    for b in range(0, batch, 23):
+    for b in [b for b in range(0, batch, 23) if b not in (7, 11)]:
         assert np.max(np.abs(inv[b] @ A[b] - np.eye(N))) < tol, b
         ref = linalg.log_determinant(*linalg.lu_construct(A[b]))
         assert abs(ld[b].real - ref.real) < 1e-11 * max(1, abs(ref.real)), b
@@ -149,7 +157,7 @@ def test_gj_inverse_large_batch(N, precision):
     few = gj_inverse(A[:5], precision=precision)
     if precision:
         assert np.array_equal(few[0], inv[:5])
-    assert np.max(np.abs(few[1] - ld[:5])) < 1e-12
+    assert np.max(np.abs(few[1] - ld[:5])) < 1e-12 * N
 
 
 def test_gj_pivoting_cases():
