@@ -12,8 +12,6 @@ unsigned long long afqmc_seed_mix_host(unsigned long long seed) {
 
 namespace {
 
-constexpr double PI = 3.14159265358979323846;
-
 __device__ double block_sum_d(double v, double* sh) {
   const int tid = threadIdx.x;
   v = wave_sum(v);

@@ -160,15 +160,22 @@ int afqmc_ctx_synchronize(afqmc_ctx ctx) {
 
 static int resolve_events(afqmc_ctx ctx) {
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  // every pending pair is read and released; the first failure is reported after all of them
+  hipError_t first = hipSuccess;
   for (auto& e : ctx->events) {
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, e.second.first, e.second.second));
-    ctx->total_ms[e.first] += ms;
-    ctx->count[e.first] += 1;
-    hipEventDestroy(e.second.first);
-    hipEventDestroy(e.second.second);
+    const hipError_t err = hipEventElapsedTime(&ms, e.second.first, e.second.second);
+    if (err == hipSuccess) {
+      ctx->total_ms[e.first] += ms;
+      ctx->count[e.first] += 1;
+    } else if (first == hipSuccess) {
+      first = err;
+    }
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
   }
   ctx->events.clear();
+  HIP_TRY(first);
   return 0;
 }
 

@@ -64,15 +64,23 @@ struct RegionTimer {
   hipEvent_t a = nullptr, b = nullptr;
   RegionTimer(afqmc_ctx c, int r) : ctx(c), reg(r) {
     if (ctx && ctx->timing && !capturing()) {
-      hipEventCreate(&a);
-      hipEventCreate(&b);
-      hipEventRecord(a, ctx->stream);
+      // a region whose events cannot be made or recorded is left out of the timing
+      if (hipEventCreate(&a) != hipSuccess) a = nullptr;
+      if (a && hipEventCreate(&b) != hipSuccess) b = nullptr;
+      if (a && (!b || hipEventRecord(a, ctx->stream) != hipSuccess)) {
+        (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        a = b = nullptr;
+      }
     }
   }
   ~RegionTimer() {
-    if (a) {
-      hipEventRecord(b, ctx->stream);
+    if (!a) return;
+    if (hipEventRecord(b, ctx->stream) == hipSuccess) {
       ctx->events.push_back({reg, {a, b}});
+    } else {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
     }
   }
 };
@@ -80,18 +88,19 @@ struct RegionTimer {
 // Selects the context's device for the lifetime of a C entry point.
 struct ScopedDevice {
   int prev = 0;
+  // device selection errors surface at the entry point's first HIP call on that device
   explicit ScopedDevice(afqmc_ctx ctx) {
-    hipGetDevice(&prev);
-    if (ctx && ctx->device != prev) hipSetDevice(ctx->device);
+    (void)hipGetDevice(&prev);
+    if (ctx && ctx->device != prev) (void)hipSetDevice(ctx->device);
   }
   explicit ScopedDevice(int device) {
-    hipGetDevice(&prev);
-    if (device != prev) hipSetDevice(device);
+    (void)hipGetDevice(&prev);
+    if (device != prev) (void)hipSetDevice(device);
   }
   ~ScopedDevice() {
     int cur = 0;
-    hipGetDevice(&cur);
-    if (cur != prev) hipSetDevice(prev);
+    (void)hipGetDevice(&cur);
+    if (cur != prev) (void)hipSetDevice(prev);
   }
 };
 
