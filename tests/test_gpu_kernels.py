@@ -129,17 +129,18 @@ def test_gj_inverse_fp32_storage(N):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("N", [100, 112])
+@pytest.mark.parametrize("N", [17, 40, 64, 100, 112])
 def test_gj_inverse_large_batch(N, precision):
-    """More matrices than CUs take the 8-wave kernel with part of the matrix in LDS: the same bar
-    as the single-workgroup kernel, and FP32 outputs identical to it bit for bit."""
+    """More matrices than CUs take the kernels with fewer waves per matrix (4 waves up to 64
+    columns; 8 waves with part of the matrix in LDS for 97..112): the same bar as with a few
+    matrices, and FP32 outputs identical to those bit for bit."""
     rng = np.random.default_rng(30 + N + precision)
     batch = 600
     A = crand(rng, batch, N, N) + 0.5 * np.eye(N)
-    # an exactly singular matrix (a zero row in the second row group) and a permutation matrix
-    # with pivots in both row groups
+    # an exactly singular matrix (a zero last row) and a permutation matrix (rows rotated by N/2,
+    # so that above 64 rows the pivots come from both row groups)
     A[7][N - 1] = 0
-    A[11] = np.eye(N)[np.r_[64:N, 0:64]]
+    A[11] = np.eye(N)[np.r_[N // 2:N, 0:N // 2]]
     if precision:
         A = A.astype(np.complex64).astype(np.complex128)
     inv, ld, info = gj_inverse(A, precision=precision)
