@@ -15,10 +15,10 @@
 //   -pivot (the exchange step is linear in each row), which is divided out when the inverse is
 //   written.  One barrier per step.  For 129 to 160 rows the third row group (rows 128..159,
 //   lanes 0..31) lives in LDS, each wave holding its own columns of it: thirty complex registers
-//   per lane would not fit the 128 a 16-wave workgroup allows.  For 97 to 112 rows and more
-//   matrices than CUs, 8 waves own 14 columns each and the last 7 of them keep their second row
-//   group in LDS: 128 registers and 68 KB of LDS per matrix let two matrices share a CU, and
-//   each hides the other's per-step synchronisation.  Up to 64 rows, 8 or 4 waves per matrix
+//   per lane would not fit the 128 a 16-wave workgroup allows.  For 65 to 112 rows and more
+//   matrices than CUs, 8 waves own 12 or 14 columns each and the last half of them keep their
+//   second row group in LDS: up to 128 registers and 68 KB of LDS per matrix let two matrices
+//   share a CU, and each hides the other's per-step synchronisation.  Up to 64 rows, 8 or 4 waves per matrix
 //   (one row group in registers) let several matrices share a CU.
 // chol_inverse: S = R^H R (upper R, positive real diagonal) and R^{-1} in one sweep, row
 //   formulation in one array: after the Jacobi scaling S' = D^-1/2 S D^-1/2, step k leaves
@@ -583,11 +583,11 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
     const bool many = a.batch > cu_count();
     if (a.N <= 32) return many ? run_gj<T, 1, 8, 4, 8>(a, s) : run_gj<T, 1, 4, 8, 4>(a, s);
     if (a.N <= 64) return many ? run_gj<T, 1, 16, 4, 16>(a, s) : run_gj<T, 1, 8, 8, 8>(a, s);
-    if (a.N <= 96) return run_gj<T, 2, 6>(a, s);
+    if (a.N <= 96) return many ? run_gj<T, 2, 12, 8, 6>(a, s) : run_gj<T, 2, 6>(a, s);
     // 8 waves per matrix with half of the second row group in LDS: two matrices share a CU and
     // each hides the other's per-step synchronisation (2048 x 112^2: 1.14 against 1.36 ms), which
     // pays once there are more matrices than CUs
-    if (a.N <= 112) return a.batch > cu_count() ? run_gj<T, 2, 14, 8, 7>(a, s) : run_gj<T, 2, 7>(a, s);
+    if (a.N <= 112) return many ? run_gj<T, 2, 14, 8, 7>(a, s) : run_gj<T, 2, 7>(a, s);
     if (a.N <= 128) return run_gj<T, 2, 8>(a, s);
     return run_gj<T, 3, 10, WAVES, 0>(a, s);
   }

@@ -129,10 +129,10 @@ def test_gj_inverse_fp32_storage(N):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("N", [17, 40, 64, 100, 112])
+@pytest.mark.parametrize("N", [17, 40, 64, 80, 100, 112])
 def test_gj_inverse_large_batch(N, precision):
     """More matrices than CUs take the kernels with fewer waves per matrix (4 waves up to 64
-    columns; 8 waves with part of the matrix in LDS for 97..112): the same bar as with a few
+    columns; 8 waves with part of the matrix in LDS for 65..112): the same bar as with a few
     matrices, and FP32 outputs identical to those bit for bit."""
     rng = np.random.default_rng(30 + N + precision)
     batch = 600
