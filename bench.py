@@ -76,7 +76,7 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, ke
     prop.skip_uniforms(model.L)
     prop.onebody(ws, 1)
     # GEMM tile shapes picked by the autotuner on these walkers' own shapes (state unchanged)
-    tiles, _ = prop.autotune(ws, reps=3)
+    tiles, _ = prop.autotune(ws, reps=5)
     tiles = [CPPropagator.TILES[t] for t in tiles]
     ctx.synchronize()
     idx = [0, 0]
