@@ -116,19 +116,32 @@ __global__ __launch_bounds__(256) void taylor_acc_kernel(long long n, const cplx
   }
 }
 
-template <typename T>
+// tr(D_g D_g) of each Ns x Ns block D_g (rows g Ns .. g Ns + Ns - 1 of the walker's D).  With
+// STAGE the block is first read column by column (contiguous rows, coalesced) into LDS with a
+// padded stride, so the transposed operand is an LDS read rather than a global load G Ns
+// elements apart; both forms add the same products in the same order.
+template <typename T, bool STAGE>
 __global__ __launch_bounds__(256) void exchange_kernel(int G, int Ns, const cplx<T>* __restrict__ D, zd* ex,
                                                         int accumulate) {
   const int w = blockIdx.y;
   const int g = blockIdx.x;
   const long long ld = (long long)G * Ns;
-  const cplx<T>* d = D + (long long)w * ld * Ns;
+  const cplx<T>* d = D + (long long)w * ld * Ns + (long long)g * Ns;
   __shared__ double sh[8];
+  extern __shared__ unsigned char stage_mem[];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(stage_mem);  // tile[c (Ns + 1) + r] = D_g(r, c)
+  if (STAGE) {
+    for (int e = threadIdx.x; e < Ns * Ns; e += blockDim.x) {
+      const int r = e % Ns, c = e / Ns;
+      tile[c * (Ns + 1) + r] = d[r + (long long)c * ld];
+    }
+    __syncthreads();
+  }
   zd s = cmk<double>(0, 0);
   for (int e = threadIdx.x; e < Ns * Ns; e += blockDim.x) {
     const int i = e / Ns, j = e % Ns;
-    const zd a = to_zd(d[(long long)g * Ns + j + (long long)i * ld]);
-    const zd b = to_zd(d[(long long)g * Ns + i + (long long)j * ld]);
+    const zd a = to_zd(STAGE ? tile[i * (Ns + 1) + j] : d[j + (long long)i * ld]);
+    const zd b = to_zd(STAGE ? tile[j * (Ns + 1) + i] : d[i + (long long)j * ld]);
     s = cadd(s, cmul(a, b));
   }
   s = bsum_z(s, sh);
@@ -237,7 +250,15 @@ template int launch_taylor_accumulate<float>(int, long long, const void*, void*,
 
 template <typename T>
 int launch_exchange_trace(int nw, int G, int Ns, const void* D, zd* ex, int accumulate, hipStream_t s) {
-  hipLaunchKernelGGL(exchange_kernel<T>, dim3(G, nw), dim3(256), 0, s, G, Ns, (const cplx<T>*)D, ex, accumulate);
+  // blocks whose padded copy fits 64 KB of LDS are staged (Ns up to 63 in FP64, 90 in FP32);
+  // larger ones read global memory
+  const size_t stage = (size_t)Ns * (Ns + 1) * sizeof(cplx<T>);
+  if (stage <= 65536)
+    hipLaunchKernelGGL((exchange_kernel<T, true>), dim3(G, nw), dim3(256), stage, s, G, Ns, (const cplx<T>*)D, ex,
+                       accumulate);
+  else
+    hipLaunchKernelGGL((exchange_kernel<T, false>), dim3(G, nw), dim3(256), 0, s, G, Ns, (const cplx<T>*)D, ex,
+                       accumulate);
   HIP_LAUNCH_CHECK();
   return 0;
 }
