@@ -146,7 +146,7 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(4))) vo
   const bool llane = lane < LL;
   // whether element (r, c) is a register (folded at compile time once the loops are unrolled)
   auto reg = [](int r, int c) { return r < RL || c < NL; };
-  auto lds = [&](int c, int l) -> zd& { return gl[wv][LG && c >= NL ? c - NL : 0][LG ? l : 0]; };
+  auto lds = [&](int c, int l) -> zd& { return gl[LG ? wv : 0][LG && c >= NL ? c - NL : 0][LG ? l : 0]; };
   zd x[NR][NC];
 #pragma unroll
   for (int r = 0; r < NR; ++r)
@@ -545,7 +545,8 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
   }
 }
 
-// compute units of the current device (cached per device)
+// compute units of the current device (cached per device; threads racing on the first call all
+// store the same value)
 int cu_count() {
   static int cached[64] = {0};
   int dev = 0;
