@@ -349,11 +349,12 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
   REQUIRE(d->decomp_type >= 0 && d->decomp_type <= 3, "Can not find the matched decompType! %d", d->decomp_type);
   REQUIRE(d->force_type == 0 || d->force_type == 1, "Do not know method.forceType %d", d->force_type);
   REQUIRE(d->force_cap >= 0.0, "forceCap must be positive or zero!");
+  REQUIRE(d->precision >= 0 && d->precision <= 2, "afqmc_cp_create: precision must be 0, 1 or 2");
   ScopedDevice sd(ctx);
   afqmc_cp cp = new afqmc_cp_s();
   cp->ctx = ctx;
   cp->device = ctx->device;
-  cp->L = d->L; cp->N = d->N; cp->fp32 = d->precision ? 1 : 0;
+  cp->L = d->L; cp->N = d->N; cp->fp32 = d->precision ? 1 : 0; cp->split_bf16 = d->precision == 2 ? 1 : 0;
   cp->decomp = d->decomp_type; cp->force_type = d->force_type; cp->dt = d->dt; cp->cap = d->force_cap;
   cp->maxw = d->max_walkers;
   const int L = d->L;

@@ -55,6 +55,7 @@ int zgemm_t(afqmc_ctx ctx, int opa, int M, int N, int K, int batch, const double
   a.scale_group = 1;
   a.herm = flags & 1;
   a.b_upper = (flags >> 1) & 1;
+  a.split_bf16 = (flags >> 2) & 1;
   a.tile = tile;
   if (scale) {
     // one K-vector per column group of scale_group columns (per batch: N / scale_group groups)
@@ -110,7 +111,7 @@ int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K,
                      long long strideA, const double* B, const double* scale, int scale_group, int flags, int tile,
                      double* C) {
   REQUIRE(ctx && A && B && C, "null argument");
-  REQUIRE(flags >= 0 && flags <= 3, "flags: bit 0 Hermitian result, bit 1 upper-triangular B");
+  REQUIRE(flags >= 0 && flags <= 7, "flags: bit 0 Hermitian result, bit 1 upper-triangular B, bit 2 split-BF16 products");
   REQUIRE(M > 0 && N > 0 && K > 0 && batch > 0, "sizes must be positive");
   REQUIRE(opa >= 0 && opa <= 3, "opa must be 0 (N), 1 (T), 2 (C) or 3 (N with real A)");
   REQUIRE(!scale || opa == OP_N, "row scaling is supported with op(A) = N");

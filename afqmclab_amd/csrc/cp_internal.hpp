@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <complex>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -19,6 +20,7 @@ struct afqmc_cp_s {
   afqmc_ctx ctx = nullptr;
   int device = 0;  // destroy must not touch ctx, which may already be gone
   int L = 0, N = 0, Nup = 0, Ndn = 0, nblk = 1, dense = 1, fp32 = 0;
+  int split_bf16 = 0;  // the propagation GEMM in split-BF16 products (precision 2)
   int Mb[2] = {0, 0}, Nb[2] = {0, 0}, row0[2] = {0, 0}, col0[2] = {0, 0};
   int decomp = 1, force_type = 0, diag_only = 1, maxw = 0;
   double dt = 0.01, cap = 1.5, log_half_L = 0.0, dtUSum = 0.0;
@@ -160,6 +162,7 @@ int propagate_gemm(afqmc_cp cp, afqmc_walkers w, int which, DevBuf<char>* in, De
       a.scale = cp->scale[b].p; a.scale_group = cp->Nb[b]; a.scale_strideG = cp->Mb[b]; a.scale_strideB = 0;
     }
     a.tile = cp->tile[0];
+    a.split_bf16 = std::is_same<T, float>::value ? cp->split_bf16 : 0;
     int rc = cp_gemm<T>(cp, OP_N, EPI_STORE, a);
     if (rc) return rc;
   }

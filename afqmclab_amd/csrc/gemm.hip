@@ -403,6 +403,179 @@ void cgemm_kernel(GemmArgs p) {
   }
 }
 
+
+// Split-BF16 complex GEMM, op N, plain store (see GemmArgs::split_bf16): 128 x 64 output tile, 4
+// waves of 64 x 32, K tiles of 16 (one v_mfma_f32_16x16x16bf16_1k step).  The 3M form as in
+// cgemm_kernel: P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B), each real
+// product from three BF16 products (lo*hi, hi*lo, hi*hi).  Operands are split when the K tile is
+// stored to LDS, as six BF16 planes per operand, k contiguous: a lane reads its four k values
+// of one row with one 8-byte LDS load.  MFMA operand layout: lane l holds A[l % 16][4 (l / 16) + i]
+// and B[4 (l / 16) + i][l % 16]; the accumulator as in the FP32 16x16x4 form.
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ void bf16_split(float x, unsigned short& hi, unsigned short& lo) {
+  hi = bf16_rne(x);
+  lo = bf16_rne(x - __uint_as_float((unsigned)hi << 16));
+}
+
+template <bool SCALE>
+__global__ __launch_bounds__(256) void bf16x3_gemm_kernel(GemmArgs p) {
+  constexpr int BM = 128, BN = 64, BK = 16, WM = 2, NT = 256;
+  constexpr int TM = BM / WM, TN = BN / 2, FM = TM / 16, FN = TN / 16;
+  constexpr int A_ELEMS = BM * BK / NT, B_ELEMS = BK * BN / NT;
+  typedef cplx<float> ct;
+  // [plane: re hi, re lo, im hi, im lo, sum hi, sum lo][row][k]
+  __shared__ unsigned short As[6][BM][BK];
+  __shared__ unsigned short Bs[6][BN][BK];
+
+  int bx, by, b;
+  {
+    const unsigned long long gx = gridDim.x, gy = gridDim.y;
+    const unsigned long long total = gx * gy * gridDim.z;
+    const unsigned long long L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned long long per = total / 8;
+    const unsigned long long lg = (L < per * 8) ? (L % 8) * per + L / 8 : L;
+    bx = (int)(lg % gx);
+    const unsigned long long r = lg / gx;
+    by = (int)(r % gy);
+    b = (int)(r / gy);
+  }
+  if (p.alive && !p.alive[b]) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int m0 = bx * BM, n0 = by * BN;
+  const ct* __restrict__ A = reinterpret_cast<const ct*>(p.A) + (long long)b * p.strideA;
+  const ct* __restrict__ Bm = reinterpret_cast<const ct*>(p.B) + (long long)b * p.strideB;
+  const ct* __restrict__ S = SCALE ? reinterpret_cast<const ct*>(p.scale) + (long long)b * p.scale_strideB : nullptr;
+
+  typedef mfma_traits<float>::acc_t acc_t;
+  acc_t p1[FM][FN], p2[FM][FN], p3[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      p1[i][j] = acc_t{0, 0, 0, 0};
+      p2[i][j] = acc_t{0, 0, 0, 0};
+      p3[i][j] = acc_t{0, 0, 0, 0};
+    }
+
+  // load addressing as in cgemm_kernel: per-thread bases plus uniform strides
+  const int a_m = tid % BM, a_k = tid / BM;
+  const long long a_base = (long long)(m0 + a_m) + (long long)a_k * p.lda;
+  const long long a_estride = (long long)(NT / BM) * p.lda;
+  const int b_k = tid % BK, b_n = tid / BK;
+  constexpr int BE = NT / BK;
+  const long long b_base = (long long)b_k + (long long)(n0 + b_n) * p.ldb;
+  const long long b_estride = (long long)BE * p.ldb;
+  long long s_base[SCALE ? B_ELEMS : 1];
+  if (SCALE) {
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e)
+      s_base[e] = (long long)((n0 + b_n + e * BE) / p.scale_group) * p.scale_strideG + b_k;
+  }
+  ct ra[A_ELEMS], rb[B_ELEMS], rs[SCALE ? B_ELEMS : 1];
+  bool ina[A_ELEMS], inb[B_ELEMS];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < A_ELEMS; ++e) {
+      const int gm = m0 + a_m, gk = k0 + a_k + e * (NT / BM);
+      const bool in = gm < p.M && gk < p.K;
+      ina[e] = in;
+      ra[e] = A[in ? a_base + e * a_estride + (long long)k0 * p.lda : 0];
+    }
+    const bool bk_in = k0 + b_k < p.K;
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      const bool in = bk_in && n0 + b_n + e * BE < p.N;
+      inb[e] = in;
+      rb[e] = Bm[in ? b_base + e * b_estride + k0 : 0];
+      if (SCALE) rs[e] = S[in ? s_base[e] + k0 : 0];
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int e = 0; e < A_ELEMS; ++e) {
+      const int m = a_m, k = a_k + e * (NT / BM);
+      const ct v = ina[e] ? ra[e] : cmk<float>(0.f, 0.f);
+      unsigned short h, l;
+      bf16_split(v.re, h, l); As[0][m][k] = h; As[1][m][k] = l;
+      bf16_split(v.im, h, l); As[2][m][k] = h; As[3][m][k] = l;
+      bf16_split(v.re + v.im, h, l); As[4][m][k] = h; As[5][m][k] = l;
+    }
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      const int k = b_k, n = b_n + e * BE;
+      ct v = cmk<float>(0.f, 0.f);
+      if (inb[e]) v = SCALE ? cmul(rs[e], rb[e]) : rb[e];
+      unsigned short h, l;
+      bf16_split(v.re, h, l); Bs[0][n][k] = h; Bs[1][n][k] = l;
+      bf16_split(v.im, h, l); Bs[2][n][k] = h; Bs[3][n][k] = l;
+      bf16_split(v.re + v.im, h, l); Bs[4][n][k] = h; Bs[5][n][k] = l;
+    }
+  };
+
+  const int nk = (p.K + BK - 1) / BK;
+  const int lr = lane & 15, lq = lane >> 4;
+  load_tiles(0);
+  store_tiles();
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+    bf16x4_t af[FM][6], bf[FN][6];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) af[i][q] = *reinterpret_cast<const bf16x4_t*>(&As[q][wm * TM + i * 16 + lr][4 * lq]);
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) bf[j][q] = *reinterpret_cast<const bf16x4_t*>(&Bs[q][wn * TN + j * 16 + lr][4 * lq]);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][1], bf[j][0], p1[i][j], 0, 0, 0);
+        p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][0], bf[j][1], p1[i][j], 0, 0, 0);
+        p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][0], bf[j][0], p1[i][j], 0, 0, 0);
+        p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][3], bf[j][2], p2[i][j], 0, 0, 0);
+        p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][2], bf[j][3], p2[i][j], 0, 0, 0);
+        p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][2], bf[j][2], p2[i][j], 0, 0, 0);
+        p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][5], bf[j][4], p3[i][j], 0, 0, 0);
+        p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][4], bf[j][5], p3[i][j], 0, 0, 0);
+        p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i][4], bf[j][4], p3[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store_tiles();
+      __syncthreads();
+    }
+  }
+
+  ct* C = reinterpret_cast<ct*>(p.C) + (long long)b * p.strideC;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn * TN + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm * TM + i * 16 + mfma_traits<float>::row(lq, r);
+        if (gm < p.M && gn < p.N) {
+          const float x1 = p1[i][j][r], x2 = p2[i][j][r], x3 = p3[i][j][r];
+          const double xr = (double)(x1 - x2), xi = (double)((x3 - x1) - x2);
+          const double yr = p.alpha_re * xr - p.alpha_im * xi;
+          const double yi = p.alpha_re * xi + p.alpha_im * xr;
+          C[gm + (long long)gn * p.ldc] = cmk<float>((float)yr, (float)yi);
+        }
+      }
+    }
+}
+
 template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
 int run(const GemmArgs& a, hipStream_t s) {
   dim3 grid((a.M + BM - 1) / BM, (a.N + BN - 1) / BN, a.batch);
@@ -464,6 +637,20 @@ int launch_cgemm(int opa, int epi, const GemmArgs& a, hipStream_t s) {
   REQUIRE(!a.herm || (opa == OP_C && epi == EPI_STORE && !a.beta_one && a.alpha_im == 0.0 && a.M == a.N &&
                       gemm_tile_square(a.tile)),
           "cgemm: a Hermitian result needs op C, a square tile, M == N, real alpha and a plain store");
+  REQUIRE(!a.split_bf16 || (std::is_same<T, float>::value && opa == OP_N && epi == EPI_STORE && !a.beta_one &&
+                            !a.herm && !a.b_upper),
+          "cgemm: split_bf16 takes FP32, op N and a plain store");
+  if (a.split_bf16) {
+    dim3 grid((a.M + 127) / 128, (a.N + 63) / 64, a.batch);
+    if (grid.y > 65535 || grid.z > 65535) {
+      afqmc_set_error("cgemm grid too large (%u, %u, %u)", grid.x, grid.y, grid.z);
+      return -1;
+    }
+    if (sc) hipLaunchKernelGGL(bf16x3_gemm_kernel<true>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(bf16x3_gemm_kernel<false>, grid, dim3(256), 0, s, a);
+    HIP_LAUNCH_CHECK();
+    return 0;
+  }
   if (epi == EPI_STORE) {
     if (opa == OP_N) return sc ? dispatch_tile<T, OP_N, EPI_STORE, true>(a, s) : dispatch_tile<T, OP_N, EPI_STORE, false>(a, s);
     if (opa == OP_T) return dispatch_tile<T, OP_T, EPI_STORE, false>(a, s);

@@ -40,6 +40,10 @@ struct GemmArgs {
   // (square tiles only).  b_upper: B is upper triangular, so the K loop of an output column tile
   // stops at its last column.
   int herm, b_upper;
+  // split_bf16 (FP32, op N, plain store only): each FP32 operand split into two BF16 parts and
+  // each real product formed as hi*hi + hi*lo + lo*hi with BF16 MFMAs accumulating in FP32
+  // (fixed 128x64 tile); about 1e-6 of sum |a b| instead of FP32's 1e-7
+  int split_bf16;
 };
 
 // Output tile shapes of the MFMA GEMM: the autotuner picks one per call site.

@@ -459,7 +459,9 @@ def main():
     ap.add_argument("--nup", type=int, default=112)
     ap.add_argument("--U", type=float, default=8.0)
     ap.add_argument("--seed", type=int, default=985456376)
-    ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--paths", default="fp64,mixed",
+                    help="comma-separated: fp64, mixed, split_bf16 (experimental: the propagation GEMM in "
+                         "split-BF16 products)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-walkers", type=int, default=2, help="walkers per CPU worker process")
     ap.add_argument("--no-cpu", action="store_true")
@@ -498,7 +500,7 @@ def main():
             f"({copy_cal['GB_per_s']:.0f} GB/s read+write)")
     results = {}
     for path in args.paths.split(","):
-        precision = 1 if path == "mixed" else 0
+        precision = {"fp64": 0, "mixed": 1, "split_bf16": 2}[path]
         log(f"[bench] path {path}: {args.walkers} walkers, {args.warmup} warmup + {args.steps} timed steps")
         results[path] = run_path(ctx, model, phi, nup, precision, args.walkers, args.steps, args.warmup, args.seed,
                                  sched, keep_walkers=(path == "fp64"))

@@ -71,7 +71,8 @@ typedef struct {
                             rows L..2L-1 x cols Nup..N-1), K and the HS field do not mix spins;
                             stored and multiplied as two blocks.  0: dense 2L x N. */
   int precision;         /* 0: complex FP64 throughout; 1: complex FP32 walkers and GEMMs, FP64 LU,
-                            log-determinants, QR, weights and energies */
+                            log-determinants, QR, weights and energies; 2: as 1, with the
+                            propagation GEMM in split-BF16 products (experimental) */
   int decomp_type;       /* NiupNidn decompType: 0 densityCharge, 1 densitySpin, 2 hopCharge, 3 hopSpin */
   int force_type;        /* 0 dynamicForce, 1 constForce */
   double dt;
@@ -313,8 +314,9 @@ int afqmc_ph_adjust_ET(afqmc_ph ph);
  * QR are what they are checked against. */
 /* C_b = op(A_b) diag(scale) B_b; A M x K (op N) or K x M (op T/C), strideA = 0 shares A;
  * opa = 3: op N with A real (A then holds real doubles).  flags bit 0: the result is Hermitian
- * (op C, A = B; only tiles on or above the diagonal computed), bit 1: B upper triangular; tile: a
- * GEMM tile shape (as afqmc_cp_set_tiles) */
+ * (op C, A = B; only tiles on or above the diagonal computed), bit 1: B upper triangular, bit 2:
+ * split-BF16 products (precision 1, op N only; the tile is then a fixed 128 x 64); tile: a GEMM
+ * tile shape (as afqmc_cp_set_tiles) */
 int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K, int batch, const double* A,
                      long long strideA, const double* B, const double* scale, int scale_group, int flags, int tile,
                      double* C);

@@ -27,7 +27,7 @@ def crand(rng, *shape):
 
 
 def zgemm(opa, A, B, precision=0, batch=1, strideA=0, scale=None, scale_group=1, herm=False, b_upper=False,
-          tile=0):
+          tile=0, split_bf16=False):
     """C_b = op(A_b) diag(scale) B_b via the device GEMM.  A, B given per batch as 3-D arrays
     (batch, rows, cols) or 2-D for shared A.  herm: the result is Hermitian (op C with A = B, only
     the upper tiles computed); b_upper: B is upper triangular."""
@@ -51,7 +51,8 @@ def zgemm(opa, A, B, precision=0, batch=1, strideA=0, scale=None, scale_group=1,
     Cbuf = np.zeros((batch, N, M), dtype=np.complex128)
     sbuf = None if scale is None else np.ascontiguousarray(scale, dtype=np.complex128)
     check(lib().afqmc_test_zgemm(ctx().h, precision, opa, M, N, K, batch, dptr(Abuf), strideA, dptr(Bbuf),
-                                 dptr(sbuf), scale_group, int(herm) | (int(b_upper) << 1), tile, dptr(Cbuf)))
+                                 dptr(sbuf), scale_group, int(herm) | (int(b_upper) << 1) | (int(split_bf16) << 2),
+                                 tile, dptr(Cbuf)))
     return np.transpose(Cbuf, (0, 2, 1))
 
 

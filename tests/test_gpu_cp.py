@@ -196,6 +196,33 @@ def test_step_matches_oracle(mode, decomp):
 
 
 @pytest.mark.parametrize("mode", ["dense", "blocks"])
+def test_step_split_bf16(mode):
+    """Precision 2 (the propagation GEMM in split-BF16 products, all else as the mixed path): the
+    HS fields of one step equal the oracle's, and the propagated walkers agree to 1e-4 of their
+    largest element, the mixed path's bar (the split products carry up to about 2^-16 each)."""
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode, nw=10, precision=2)
+    rng = np.random.default_rng(9)
+    u = rng.uniform(0, 1, model.L * ws.nw)
+    prop.set_uniforms(u)
+    prop.set_ET(-7.5)
+    f_dev, aux_dev = prop.step(ws, record=True)
+    out, lw, al = ws.download()
+    om_method = OracleMethod(dt=method.dt, decompType=method.decompType, walkerSizePerThread=ws.nw, ET=-7.5,
+                             thermalSize=0, ETAdjustMaxSize=0)
+    drv = OracleCP(om_method, om, ListStream(u), phiT=SD(wf=phi))
+    drv.walker = [SD(wf=wfs[i], logw=logw[i]) for i in range(ws.nw)]
+    drv.walkerIsAlive = [True] * ws.nw
+    drv.projectExpMinusDtKExpMinusDtV()
+    recs = {r[0]: r for r in drv.records}
+    for i in range(ws.nw):
+        _, f_ref, aux_ref = recs[i]
+        assert np.array_equal(aux_dev[i], aux_ref)
+        ref = drv.walker[i]
+        assert np.max(np.abs(out[i] - ref.wf)) < 1e-4 * np.max(np.abs(ref.wf))
+        close_log(lw[i], ref.logw, rtol=1e-5)
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
 def test_pop_control_matches_oracle(mode):
     model, om, phi, prop, ws, wfs, logw, method = make_case(mode, nw=16)
     rng = np.random.default_rng(4)

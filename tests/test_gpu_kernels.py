@@ -76,6 +76,28 @@ def test_zgemm_upper_triangular_b(precision, tile):
         assert np.max(np.abs(tri[b] - A[b] @ B[b])) < TOL[precision] * np.sqrt(N) * 4
 
 
+@pytest.mark.parametrize("shape", [(256, 229, 256), (130, 70, 97), (16, 8, 5)])
+def test_zgemm_split_bf16(shape):
+    """Split-BF16 products (each FP32 operand as two BF16 parts, hi*hi + hi*lo + lo*hi): within
+    3e-5 of sum |a||b| per element (each product carries up to about 2^-16 from the split, and the
+    3M form adds three real products' errors), against the
+    FP64 product, with and without the fused row scaling; ragged edges in every dimension."""
+    M, N, K = shape
+    rng = np.random.default_rng(M + N + K)
+    A = crand(rng, M, K)
+    B = crand(rng, 2, K, N)
+    C = zgemm(0, A, B, 1, split_bf16=True)
+    for b in range(2):
+        bound = np.abs(A) @ np.abs(B[b])
+        assert np.max(np.abs(C[b] - A @ B[b]) / bound) < 3e-5
+    s = crand(rng, 2, K)
+    C = zgemm(0, A, B, 1, scale=s, scale_group=N, split_bf16=True)
+    for b in range(2):
+        ref = A @ (s[b][:, None] * B[b])
+        bound = np.abs(A) @ np.abs(s[b][:, None] * B[b])
+        assert np.max(np.abs(C[b] - ref) / bound) < 3e-5
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_zgemm_batched_strided_and_scaled(precision):
     rng = np.random.default_rng(3)
