@@ -407,10 +407,12 @@ void cgemm_kernel(GemmArgs p) {
 // Split-BF16 complex GEMM, op N, plain store (see GemmArgs::split_bf16): 128 x 64 output tile, 4
 // waves of 64 x 32, K tiles of 16 (one v_mfma_f32_16x16x16bf16_1k step).  The 3M form as in
 // cgemm_kernel: P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B), each real
-// product from three BF16 products (lo*hi, hi*lo, hi*hi).  Operands are split when the K tile is
-// stored to LDS, as six BF16 planes per operand, k contiguous: a lane reads its four k values
-// of one row with one 8-byte LDS load.  MFMA operand layout: lane l holds A[l % 16][4 (l / 16) + i]
-// and B[4 (l / 16) + i][l % 16]; the accumulator as in the FP32 16x16x4 form.
+// product from three BF16 products (lo*hi, hi*lo, hi*hi).  Each thread loads runs of four
+// consecutive k of one row (A) or column (B), splits them once (re, im and re + im, each into hi
+// and lo BF16 parts) and stores six packed 8-byte planes, k contiguous; a lane then reads each MFMA
+// operand (four k of one row) with one 8-byte LDS load.  MFMA operand layout: lane l holds
+// A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]; the accumulator as in the FP32 16x16x4
+// form.
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short bf16_rne(float x) {
@@ -418,20 +420,26 @@ __device__ __forceinline__ unsigned short bf16_rne(float x) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (unsigned short)(u >> 16);
 }
-__device__ __forceinline__ void bf16_split(float x, unsigned short& hi, unsigned short& lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - __uint_as_float((unsigned)hi << 16));
+// four values into packed hi and lo BF16 parts
+__device__ __forceinline__ void bf16_split4(const float* x, bf16x4_t& hi, bf16x4_t& lo) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const unsigned short h = bf16_rne(x[t]);
+    hi[t] = (short)h;
+    lo[t] = (short)bf16_rne(x[t] - __uint_as_float((unsigned)h << 16));
+  }
 }
 
 template <bool SCALE>
 __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(GemmArgs p) {
   constexpr int BM = 128, BN = 64, BK = 16, WM = 2, NT = 256;
   constexpr int TM = BM / WM, TN = BN / 2, FM = TM / 16, FN = TN / 16;
-  constexpr int A_ELEMS = BM * BK / NT, B_ELEMS = BK * BN / NT;
+  constexpr int AP = BM * (BK / 4) / NT;  // runs of four k per thread: A 2, B 1
+  static_assert(BN * (BK / 4) == NT, "one B run per thread");
   typedef cplx<float> ct;
   // [plane: re hi, re lo, im hi, im lo, sum hi, sum lo][row][k]
-  __shared__ unsigned short As[6][BM][BK];
-  __shared__ unsigned short Bs[6][BN][BK];
+  __shared__ bf16x4_t As[6][BM][BK / 4];
+  __shared__ bf16x4_t Bs[6][BN][BK / 4];
 
   int bx, by, b;
   {
@@ -464,59 +472,58 @@ __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(GemmArgs p) {
       p3[i][j] = acc_t{0, 0, 0, 0};
     }
 
-  // load addressing as in cgemm_kernel: per-thread bases plus uniform strides
-  const int a_m = tid % BM, a_k = tid / BM;
-  const long long a_base = (long long)(m0 + a_m) + (long long)a_k * p.lda;
-  const long long a_estride = (long long)(NT / BM) * p.lda;
-  const int b_k = tid % BK, b_n = tid / BK;
-  constexpr int BE = NT / BK;
-  const long long b_base = (long long)b_k + (long long)(n0 + b_n) * p.ldb;
-  const long long b_estride = (long long)BE * p.ldb;
-  long long s_base[SCALE ? B_ELEMS : 1];
-  if (SCALE) {
-#pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e)
-      s_base[e] = (long long)((n0 + b_n + e * BE) / p.scale_group) * p.scale_strideG + b_k;
-  }
-  ct ra[A_ELEMS], rb[B_ELEMS], rs[SCALE ? B_ELEMS : 1];
-  bool ina[A_ELEMS], inb[B_ELEMS];
+  // A runs: run e of a thread is row a_m, k group a_g + e * (NT / BM); B run: column b_n, k group b_g
+  const int a_m = tid % BM, a_g = tid / BM;
+  const int b_n = tid / (BK / 4), b_g = tid % (BK / 4);
+  const bool a_row_in = m0 + a_m < p.M, b_col_in = n0 + b_n < p.N;
+  const ct* a_ptr = A + (m0 + a_m);
+  const ct* b_ptr = Bm + (long long)(n0 + b_n) * p.ldb;
+  const ct* s_ptr = SCALE ? S + (long long)((n0 + b_n) / p.scale_group) * p.scale_strideG : nullptr;
+  ct ra[AP][4], rb[4], rs[SCALE ? 4 : 1];
+  bool ina[AP][4], inb[4];
   auto load_tiles = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      const int gm = m0 + a_m, gk = k0 + a_k + e * (NT / BM);
-      const bool in = gm < p.M && gk < p.K;
-      ina[e] = in;
-      ra[e] = A[in ? a_base + e * a_estride + (long long)k0 * p.lda : 0];
-    }
-    const bool bk_in = k0 + b_k < p.K;
+    for (int e = 0; e < AP; ++e)
 #pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      const bool in = bk_in && n0 + b_n + e * BE < p.N;
-      inb[e] = in;
-      rb[e] = Bm[in ? b_base + e * b_estride + k0 : 0];
-      if (SCALE) rs[e] = S[in ? s_base[e] + k0 : 0];
+      for (int t = 0; t < 4; ++t) {
+        const int gk = k0 + 4 * (a_g + e * (NT / BM)) + t;
+        const bool in = a_row_in && gk < p.K;
+        ina[e][t] = in;
+        ra[e][t] = in ? a_ptr[(long long)gk * p.lda] : A[0];
+      }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int gk = k0 + 4 * b_g + t;
+      const bool in = b_col_in && gk < p.K;
+      inb[t] = in;
+      rb[t] = in ? b_ptr[gk] : Bm[0];
+      if (SCALE) rs[t] = in ? s_ptr[gk] : S[0];
     }
   };
   auto store_tiles = [&]() {
 #pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      const int m = a_m, k = a_k + e * (NT / BM);
-      const ct v = ina[e] ? ra[e] : cmk<float>(0.f, 0.f);
-      unsigned short h, l;
-      bf16_split(v.re, h, l); As[0][m][k] = h; As[1][m][k] = l;
-      bf16_split(v.im, h, l); As[2][m][k] = h; As[3][m][k] = l;
-      bf16_split(v.re + v.im, h, l); As[4][m][k] = h; As[5][m][k] = l;
-    }
+    for (int e = 0; e < AP; ++e) {
+      float re[4], im[4], sm[4];
 #pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      const int k = b_k, n = b_n + e * BE;
-      ct v = cmk<float>(0.f, 0.f);
-      if (inb[e]) v = SCALE ? cmul(rs[e], rb[e]) : rb[e];
-      unsigned short h, l;
-      bf16_split(v.re, h, l); Bs[0][n][k] = h; Bs[1][n][k] = l;
-      bf16_split(v.im, h, l); Bs[2][n][k] = h; Bs[3][n][k] = l;
-      bf16_split(v.re + v.im, h, l); Bs[4][n][k] = h; Bs[5][n][k] = l;
+      for (int t = 0; t < 4; ++t) {
+        const ct v = ina[e][t] ? ra[e][t] : cmk<float>(0.f, 0.f);
+        re[t] = v.re; im[t] = v.im; sm[t] = v.re + v.im;
+      }
+      const int g = a_g + e * (NT / BM);
+      bf16_split4(re, As[0][a_m][g], As[1][a_m][g]);
+      bf16_split4(im, As[2][a_m][g], As[3][a_m][g]);
+      bf16_split4(sm, As[4][a_m][g], As[5][a_m][g]);
     }
+    float re[4], im[4], sm[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      ct v = cmk<float>(0.f, 0.f);
+      if (inb[t]) v = SCALE ? cmul(rs[t], rb[t]) : rb[t];
+      re[t] = v.re; im[t] = v.im; sm[t] = v.re + v.im;
+    }
+    bf16_split4(re, Bs[0][b_n][b_g], Bs[1][b_n][b_g]);
+    bf16_split4(im, Bs[2][b_n][b_g], Bs[3][b_n][b_g]);
+    bf16_split4(sm, Bs[4][b_n][b_g], Bs[5][b_n][b_g]);
   };
 
   const int nk = (p.K + BK - 1) / BK;
@@ -530,11 +537,11 @@ __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(GemmArgs p) {
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int q = 0; q < 6; ++q) af[i][q] = *reinterpret_cast<const bf16x4_t*>(&As[q][wm * TM + i * 16 + lr][4 * lq]);
+      for (int q = 0; q < 6; ++q) af[i][q] = As[q][wm * TM + i * 16 + lr][lq];
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
-      for (int q = 0; q < 6; ++q) bf[j][q] = *reinterpret_cast<const bf16x4_t*>(&Bs[q][wn * TN + j * 16 + lr][4 * lq]);
+      for (int q = 0; q < 6; ++q) bf[j][q] = Bs[q][wn * TN + j * 16 + lr][lq];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
