@@ -422,6 +422,7 @@ int afqmc_ph_create(afqmc_ctx ctx, const afqmc_ph_desc* d, afqmc_ph* out) {
           "afqmc_ph_create: t, chol, trial and half-rotated vectors are required");
   REQUIRE(std::max(d->Nup, d->Ndn) <= 160, "electron count per spin above 160 is not supported");
   REQUIRE(d->force_type == 0 || d->force_type == 1, "Do not know method.forceType %d", d->force_type);
+  REQUIRE(d->precision == 0 || d->precision == 1, "afqmc_ph_create: precision must be 0 or 1");
   ScopedDevice sdv(ctx);
   afqmc_ph ph = new afqmc_ph_s();
   ph->ctx = ctx; ph->device = ctx->device;

@@ -2,8 +2,11 @@
 
 Follows afqmc/utilities/populationControl/source/populationControlConfiguration.cpp
 and include/populationControl.h.  Pinned by the popCheck literals of
-populationControlTest.cpp:12-27 and by invariants (every slot count is 1
-afterwards; a slot with count >= 1 keeps itself); tests/test_oracle_comb.py.
+populationControlTest.cpp:12-27, by the reference's b_exact table
+(populationControlTest.cpp:29-45: weights 1..100 over 4 chunks, reproduced slot
+for slot from uniforms placed inside each tooth's bin) and by invariants (every
+slot count is 1 afterwards; a slot with count >= 1 keeps itself); the checks are
+in tests/test_oracle_models.py.
 """
 import numpy as np
 

@@ -10,8 +10,8 @@ comb slot).
 The stream is counter based so that the device can evaluate element ``k``
 without a sequential state: u(k) = splitmix64 finaliser of
 ``seed_mix + (k + 1) * 0x9E3779B97F4A7C15`` reduced to 53 bits.  The same
-function is implemented in the HIP library (``afqmc_uniform`` in
-afqmclab_amd/csrc/rng.hip); the two are checked against each other on the GPU.
+function is implemented in the HIP library (``afqmc_uniform_dev`` in
+afqmclab_amd/csrc/cp_kernels.hpp); the two are checked against each other on the GPU.
 """
 import numpy as np
 

@@ -95,8 +95,10 @@ def test_bp_hubbard_matches_oracle(mode, nup, ndn):
     compare_writes(dev.writes, ref.writes, 1e-9)
 
 
-def test_bp_mixed_precision_close_to_oracle():
-    """FP32 walkers and GEMMs: the same run to the mixed-precision bar (energies 1e-4 relative)."""
+@pytest.mark.parametrize("precision", [1, 2])
+def test_bp_mixed_precision_close_to_oracle(precision):
+    """FP32 walkers and GEMMs (precision 2: the propagation GEMMs of kets and bras in split-BF16
+    products): the same run to the mixed-precision bar (energies 1e-4 relative)."""
     nx, ny, nup, ndn, U = 4, 4, 5, 5, 4.0
     model = HubbardModel.square(nx, ny, nup + ndn, U)
     phi = spin_block_trial(model, nup, ndn)
@@ -104,7 +106,7 @@ def test_bp_mixed_precision_close_to_oracle():
     m = dict(dt=0.05, thermalSize=10, writeNumber=1, measureNumberPerWrite=2, measureSkipStep=4,
              walkerSizePerThread=12, decompType="densitySpin", mgsStep=4, popControlStep=5,
              ETAdjustStep=5, ETAdjustMaxSize=10, backPropagationStep=8, seed=seed)
-    dev, ref = run_pair(model, phi, m, dict(nup=nup, spin_blocks=True), seed, precision=1)
+    dev, ref = run_pair(model, phi, m, dict(nup=nup, spin_blocks=True), seed, precision=precision)
     e_dev = dev.writes[0]["HNum"] / dev.writes[0]["den"]
     e_ref = ref.writes[0]["HNum"] / ref.writes[0]["den"]
     assert abs(e_dev - e_ref) < 1e-4 * abs(e_ref)

@@ -24,7 +24,7 @@ def block_stats(energies):
     return e.mean(), e.std(ddof=1) / np.sqrt(len(e)), e
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_cp_energy_4x4_u4_matches_exact_diagonalisation(precision):
     ed = json.load(open(GOLDEN))
     model = HubbardModel.square(4, 4, 10, 4.0)

@@ -249,3 +249,22 @@ def test_comb_bit_exact_edges(case):
         ref = ocomb.popConfiguration(1, w, u)
     t, st = comb(ctx(), w, u)
     assert np.array_equal(t, ref)
+
+
+def test_comb_reference_b_exact_weights():
+    """The weights and tooth positions of the reference's b_exact table
+    (populationControlTest.cpp:29-45: weights 1..100): the device comb (one chunk) equals the
+    oracle's size-1 table, whose size-4 form reproduces b_exact slot for slot."""
+    import json
+    import os
+    from afqmclab_amd.cp import comb
+    from tests.test_oracle_models import _uniforms_for_table
+    k = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "reference_kats.json")))["popConfiguration_b_exact"]
+    w = np.array(k["weights"])
+    expect = np.array(k["table_1based"]) - 1
+    u = _uniforms_for_table(list(w), expect)
+    assert np.array_equal(ocomb.popConfiguration(k["size"], w, u), expect)
+    t, st = comb(ctx(), w, u)
+    assert np.array_equal(t, ocomb.popConfiguration(1, w, u))
+    assert np.array_equal(np.bincount(t, minlength=100), np.bincount(expect, minlength=100))

@@ -76,7 +76,114 @@ def test_popconfiguration_all_but_one_dead_and_clamp():
     assert table[L - 1] == L - 1
 
 
+def _uniforms_for_table(weights, table):
+    """Uniforms u_i (one per tooth) that make the comb produce `table`'s parent counts.
+
+    Tooth i sits at x_i = (i + u_i)/L and lands in the first bin k whose rounded cumulative sum
+    c_k satisfies x_i <= c_k, so the teeth of bin k are the number[k] teeth after those of the
+    bins before it, and u_i must lie in (L c_{k-1} - i, L c_k - i].  The midpoint of that
+    interval (clipped to [0, 1)) is returned, or None if the interval is empty."""
+    L = len(weights)
+    s = 0.0
+    for w in weights:
+        s += w
+    cum, c = [], 0.0
+    for w in weights:
+        c += w / s
+        cum.append(c)
+    number = np.bincount(np.asarray(table), minlength=L)
+    u = np.empty(L)
+    tooth = 0
+    for k in range(L):
+        for _ in range(number[k]):
+            lo = (L * cum[k - 1] - tooth) if k > 0 else 0.0
+            hi = L * cum[k] - tooth
+            lo, hi = max(lo, 0.0), min(hi, 1.0)
+            if not lo < hi:
+                return None
+            u[tooth] = 0.5 * (lo + hi)
+            tooth += 1
+    return u
+
+
+def test_popconfiguration_reproduces_reference_b_exact_table():
+    """populationControlTest.cpp:29-45: weights 1..100, 4 chunks.  The table's parent counts fix
+    one tooth interval per slot; with any uniforms inside those intervals the comb, the in-chunk
+    reassignment and the cross-chunk pass must give the reference's table slot for slot."""
+    k = KATS["popConfiguration_b_exact"]
+    w = k["weights"]
+    expect = np.array(k["table_1based"]) - 1
+    u = _uniforms_for_table(w, expect)
+    assert u is not None
+    assert np.all((u >= 0) & (u < 1))
+    table = comb.popConfiguration(k["size"], w, u)
+    assert np.array_equal(table, expect)
+    # the same counts reassigned by a scan from the top of each chunk give another table: the
+    # fixture pins the reassignment order, not only the comb's counts
+    number = list(np.bincount(expect, minlength=len(w)))
+    alt = [i if number[i] else -1 for i in range(len(w))]
+    for r in range(k["size"]):
+        lo, hi = 25 * r, 25 * (r + 1)
+        donors = [j for j in range(hi - 1, lo - 1, -1) for _ in range(max(number[j] - 1, 0))]
+        for z in range(hi - 1, lo - 1, -1):
+            if alt[z] < 0 and donors:
+                alt[z] = donors.pop(0)
+    assert not np.array_equal(np.array(alt), expect)
+
+
 # ---------------------------------------------------------------- Hubbard model
+def _hs_generator_derivative(decomp, gamma_i, i, L, phiT, W, eps=1e-6):
+    """d/dx log <phiT| B_i(x) |W> at x = 0 by central differences, where B_i(x) is the one-site
+    HS factor of NiupNidn::setTwoBodySampleMatrix (NiupNidn.cpp:252-322) without constDiag,
+    times the field-dependent part of the sample logw (NiupNidn.cpp:108-113: -gamma x for
+    densityCharge), acting on sites i (spin up) and i + L (spin down)."""
+    def logo(x):
+        g = gamma_i * x
+        B = np.eye(2 * L, dtype=complex)
+        if decomp == "densityCharge":
+            blk = np.array([[np.exp(g), 0], [0, np.exp(g)]])
+        elif decomp == "densitySpin":
+            blk = np.array([[np.exp(g), 0], [0, np.exp(-g)]])
+        elif decomp == "hopCharge":
+            blk = np.array([[np.cosh(g), np.sinh(g)], [np.sinh(g), np.cosh(g)]])
+        else:
+            blk = np.array([[np.cos(g), np.sin(g)], [-np.sin(g), np.cos(g)]])
+        idx = [i, i + L]
+        B[np.ix_(idx, idx)] = blk
+        sign, lad = np.linalg.slogdet(phiT.conj().T @ B @ W)
+        extra = -g if decomp == "densityCharge" else 0.0
+        return lad + np.log(sign) + extra
+
+    return (logo(eps) - logo(-eps)) / (2 * eps)
+
+
+@pytest.mark.parametrize("decomp", ["densityCharge", "densitySpin", "hopCharge", "hopSpin"])
+def test_cp_force_is_derivative_of_log_overlap(decomp):
+    """The force bias of HubbardSOCMeasureFixedSDSD::getForce (:73-116) is, before clipping, the
+    real part of the derivative of the log-overlap of the walker under one site's HS factor with
+    respect to the field, at zero field.  Checked by central differences of a determinant built
+    from the HS factors, on random complex spin-mixing walkers."""
+    from oracle.blocks import NiupNidn
+
+    rng = np.random.default_rng(11)
+    L, N = 5, 4
+    U = rng.uniform(1, 6, L)
+    z = np.zeros(L)
+    hs = NiupNidn(0.05, decomp, U, z, z, z, z)
+    phiT = rng.normal(size=(2 * L, N)) + 1j * rng.normal(size=(2 * L, N))
+    W = rng.normal(size=(2 * L, N)) + 1j * rng.normal(size=(2 * L, N))
+    op = SDSDOperation(SD(wf=phiT), SD(wf=W))
+    meas = HubbardSOCMeasureFixedSDSD.__new__(HubbardSOCMeasureFixedSDSD)
+    f = meas.getForce(hs, op, 1e9)
+    for i in range(L):
+        d = _hs_generator_derivative(decomp, hs.gamma[i], i, L, phiT, W)
+        assert abs(f[i] - d.real) < 1e-7 * max(1.0, abs(d))
+    # clipping at the cap
+    cap = 0.5 * np.max(np.abs(f))
+    fc = meas.getForce(hs, op, cap)
+    assert np.allclose(fc, np.clip(f, -cap, cap), rtol=0, atol=0)
+
+
 def test_hubbard_model_roundtrip(tmp_path):
     K = square_lattice_K(3, 2, 1.0, 0.2)
     m = HubbardSOC(6, 4, K, U=np.full(6, 4.0), mu=np.linspace(0, 1, 6))
