@@ -79,10 +79,17 @@ SIGNATURES = {
     "afqmc_cp_onebody": (C.c_int, [_vp, _vp, C.c_int]),
     "afqmc_cp_hs_apply": (C.c_int, [_vp, _vp, _dp, _dp]),
     "afqmc_cp_stabilize": (C.c_int, [_vp, _vp]),
+    "afqmc_cp_get_qr_stats": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "afqmc_ph_get_qr_stats": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "afqmc_comb": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip, _dp]),
+    "afqmc_walkers_download_some": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip]),
+    "afqmc_walkers_set_alive": (C.c_int, [_vp, _ip]),
+    "afqmc_cp_walkers_from_trial": (C.c_int, [_vp, _vp, _dp]),
     "afqmc_cp_apply_table": (C.c_int, [_vp, _vp, _ip]),
     "afqmc_cp_step": (C.c_int, [_vp, _vp, _dp, _ip]),
     "afqmc_cp_pop_control": (C.c_int, [_vp, _vp, _dp, _ip, _dp]),
+    "afqmc_cp_step_fields": (C.c_int, [_vp, _vp, _ip]),
+    "afqmc_cp_pop_control_table": (C.c_int, [_vp, _vp, _ip, _dp]),
     "afqmc_cp_measure": (C.c_int, [_vp, _vp]),
     "afqmc_bp_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "afqmc_bp_destroy": (C.c_int, [_vp]),

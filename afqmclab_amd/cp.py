@@ -287,6 +287,12 @@ class CPPropagator:
         lw = np.ascontiguousarray(sample_logw, dtype=np.complex128)
         check(lib().afqmc_cp_hs_apply(self.h, w.h, dptr(d), dptr(lw)))
 
+    def qr_stats(self):
+        """(walkers redone by shifted CholeskyQR, walkers marked dead as rank deficient)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().afqmc_cp_get_qr_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def stabilize(self, w):
         check(lib().afqmc_cp_stabilize(self.h, w.h))
 
@@ -312,6 +318,20 @@ class CPPropagator:
             return wt, tb, st
         check(lib().afqmc_cp_pop_control(self.h, w.h, None, None, None))
         return None
+
+    def step_fields(self, w, aux):
+        """One step with the fields given (nw x L, +-1) instead of sampled: no uniforms drawn."""
+        a = np.ascontiguousarray(aux, dtype=np.int32)
+        if a.shape != (w.nw, self.model.L):
+            raise ValueError("aux must be nw x L")
+        check(lib().afqmc_cp_step_fields(self.h, w.h, iptr(a)))
+
+    def pop_control_table(self, w, table):
+        """popControl with the comb's table given; returns the weights."""
+        t = np.ascontiguousarray(table, dtype=np.int32)
+        wt = np.zeros(w.nw)
+        check(lib().afqmc_cp_pop_control_table(self.h, w.h, iptr(t), dptr(wt)))
+        return wt
 
     def measure(self, w):
         check(lib().afqmc_cp_measure(self.h, w.h))
@@ -378,9 +398,33 @@ class WalkerSet:
         check(lib().afqmc_walkers_download(self.h, dptr(buf), dptr(lw), iptr(al)))
         return buffer_to_walkers(buf), lw, al
 
-    def set_from_trial(self, phiT, logw=0j):
-        """setWalkerFromPhiT (SDSDOperation.cpp:176-186): every walker = phiT, alive."""
+    def set_from_trial(self, phiT=None, logw=0j):
+        """setWalkerFromPhiT (SDSDOperation.cpp:176-186): every walker = phiT, alive.  Without phiT,
+        the propagator's trial is copied on the device (no host walker buffers)."""
+        if phiT is None:
+            lw = np.array([complex(logw)], dtype=np.complex128)
+            check(lib().afqmc_cp_walkers_from_trial(self.prop.h, self.h, dptr(lw)))
+            return
         self.upload([phiT] * self.nw, np.full(self.nw, complex(logw)))
+
+    def download_some(self, index):
+        """The walkers index[...] as (wf list, logw, alive)."""
+        idx = np.ascontiguousarray(index, dtype=np.int32)
+        L2, N = 2 * self.prop.model.L, self.prop.model.N
+        buf = np.zeros((len(idx), N, L2), dtype=np.complex128)
+        lw = np.zeros(len(idx), dtype=np.complex128)
+        al = np.zeros(len(idx), dtype=np.int32)
+        check(lib().afqmc_walkers_download_some(self.h, len(idx), iptr(idx), dptr(buf), dptr(lw), iptr(al)))
+        return buffer_to_walkers(buf), lw, al
+
+    def alive(self):
+        """The alive mask of every walker (no walker data is copied)."""
+        al = np.zeros(self.nw, dtype=np.int32)
+        check(lib().afqmc_walkers_download(self.h, None, None, iptr(al)))
+        return al
+
+    def set_alive(self, alive):
+        check(lib().afqmc_walkers_set_alive(self.h, iptr(np.ascontiguousarray(alive, dtype=np.int32))))
 
 
 class AfqmcConstraintPath:

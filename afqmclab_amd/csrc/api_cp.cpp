@@ -11,6 +11,7 @@
 #include <mutex>
 #include <vector>
 
+#include "cholqr.hpp"
 #include "cp_internal.hpp"
 
 // ------------------------------------------------------------------ NiupNidn constants (host)
@@ -101,7 +102,8 @@ HsSampleArgs hs_args(afqmc_cp cp, afqmc_walkers w) {
 }
 
 template <typename T>
-int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out, int* aux_dev = nullptr) {
+int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out, int* aux_dev = nullptr,
+              const int* aux_in_dev = nullptr) {
   hipStream_t s = cp->ctx->stream;
   int rc;
   REQUIRE(!capturing() || (!force_out && !aux_out), "a captured step cannot return records");
@@ -127,10 +129,12 @@ int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out, int
     h.aux_out = cp->aux_rec.p;
   }
   if (aux_dev) h.aux_out = aux_dev;
+  h.aux_in = aux_in_dev;
   {
     RegionTimer t(cp->ctx, REG_HS_SAMPLE);
     if ((rc = launch_hs_sample(h, s))) return rc;
-    if ((rc = launch_advance_counter(cp->sc.p, cp->L, s))) return rc;
+    // given fields draw no uniforms: only the walker-step count advances
+    if ((rc = launch_advance_counter(cp->sc.p, aux_in_dev ? 0 : cp->L, s))) return rc;
   }
   if (cp->diag_only) {
     RegionTimer t(cp->ctx, REG_PROPAGATE);
@@ -152,54 +156,18 @@ int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out, int
   return 0;
 }
 
-// CholeskyQR2 in FP64 of each block: W <- W R^{-1}, logw += sum log R_kk (both passes)
-// CholeskyQR in FP64 of each block: S = W^H W = R^H R, W <- W R^{-1}, logw += sum log R_kk.  A
-// second pass (CholeskyQR2) runs only for the walkers whose Q^H Q is still farther than `tol` from
-// the identity after the first; the others keep their first-pass Q.
+// SD::stabilize of each block by robust CholeskyQR in FP64 (cholqr.hpp): W <- Q, logw += sum log R_kk
 int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tmp, double tol) {
-  hipStream_t s = cp->ctx->stream;
-  int rc;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int b = 0; b < cp->nblk; ++b) {
-      const int Mb = cp->Mb[b], Nb = cp->Nb[b];
-      GemmArgs a = gemm_base<double>();
-      a.M = Nb; a.N = Nb; a.K = Mb; a.batch = w->nw;
-      a.A = wf[b].p; a.lda = Mb; a.strideA = w->wsize(b);
-      a.B = wf[b].p; a.ldb = Mb; a.strideB = w->wsize(b);
-      a.C = cp->qrS[b].p; a.ldc = Nb; a.strideC = (long long)Nb * Nb;
-      a.alive = w->alive.p;
-      a.herm = 1;
-      // measured fastest for the Gram shapes: one 128x128 tile per matrix above 64 columns
-      a.tile = Nb > 64 ? GEMM_TILE_128x128 : GEMM_TILE_64x64;
-      if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
-      const int* mask = w->alive.p;
-      if (pass == 1) {
-        if ((rc = launch_ortho_check(w->nw, Nb, (const zd*)cp->qrS[b].p, w->alive.p, tol, cp->qr_need.p, s)))
-          return rc;
-        mask = cp->qr_need.p;
-      }
-      SmallDenseArgs c;
-      memset(&c, 0, sizeof(c));
-      c.N = Nb; c.batch = w->nw; c.in = cp->qrS[b].p; c.ld_in = Nb; c.stride_in = (long long)Nb * Nb;
-      c.out = cp->qrS[b].p; c.ld_out = Nb; c.stride_out = c.stride_in;
-      c.logdet = cp->qrld[b].p; c.info = cp->qrinfo[b].p; c.nsub = Nb; c.alive = mask; c.alive_div = 1;
-      if ((rc = launch_chol_inverse<double>(c, s))) return rc;
-      // first pass: tmp = W R^{-1} for every walker, then the buffers swap; second pass: only the
-      // walkers that need it are copied to tmp and multiplied back into wf, the others stay put
-      if (pass == 1 && (rc = launch_select_alive(w->nw, w->wsize(b), sizeof(zd), mask, wf[b].p, tmp[b].p, s)))
-        return rc;
-      GemmArgs g = gemm_base<double>();
-      g.M = Mb; g.N = Nb; g.K = Nb; g.batch = w->nw;
-      g.A = pass == 0 ? wf[b].p : tmp[b].p; g.lda = Mb; g.strideA = w->wsize(b);
-      g.B = cp->qrS[b].p; g.ldb = Nb; g.strideB = (long long)Nb * Nb;
-      g.C = pass == 0 ? tmp[b].p : wf[b].p; g.ldc = Mb; g.strideC = w->wsize(b);
-      g.alive = mask;
-      g.b_upper = 1;  // R^{-1} is upper triangular
-      g.tile = GEMM_TILE_32x32;  // measured fastest for W R^{-1}
-      if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
-      if (pass == 0) std::swap(wf[b], tmp[b]);
-      if ((rc = launch_logw_add_real(w->nw, w->logw.p, mask, cp->qrld[b].p, nullptr, 0, 1.0, 0.0, s))) return rc;
-    }
+  for (int b = 0; b < cp->nblk; ++b) {
+    CholQrBlock q;
+    q.nw = w->nw; q.M = cp->Mb[b]; q.N = cp->Nb[b]; q.nsub = cp->Nb[b];
+    q.wf = &wf[b]; q.tmp = &tmp[b];
+    q.S = reinterpret_cast<zd*>(cp->qrS[b].p); q.ld = cp->qrld[b].p;
+    q.info = cp->qrinfo[b].p; q.need = cp->qr_need.p; q.flag = cp->qr_flag.p;
+    q.alive = w->alive.p; q.logw = w->logw.p;
+    q.m0 = 1.0; q.m1 = 0.0; q.tol = tol; q.sc = cp->sc.p;
+    int rc = cholqr_f64(q, cp->ctx->stream);
+    if (rc) return rc;
   }
   return 0;
 }
@@ -225,15 +193,20 @@ int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
 }
 
 template <typename T>
-int pop_control_impl(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats) {
+int pop_control_impl(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats,
+                     const int* table_in = nullptr) {
   hipStream_t s = cp->ctx->stream;
   RegionTimer t(cp->ctx, REG_COMB);
   int rc = do_overlap<T>(cp, w, false);
   if (rc) return rc;
   LogOArgs l = logo_args<T>(cp, w, LOGO_WEIGHT);
   if ((rc = launch_logo(l, s))) return rc;
-  if ((rc = launch_comb(w->nw, cp->weight.p, cp->ubuf.p, cp->sc.p, cp->table.p, cp->comb_prob.p, cp->comb_num.p, s)))
+  if (table_in) {
+    HIP_TRY(hipMemcpyAsync(cp->table.p, table_in, sizeof(int) * w->nw, hipMemcpyHostToDevice, s));
+  } else if ((rc = launch_comb(w->nw, cp->weight.p, cp->ubuf.p, cp->sc.p, cp->table.p, cp->comb_prob.p,
+                               cp->comb_num.p, s))) {
     return rc;
+  }
   void* wfp[2] = {w->wf[0].p, cp->nblk > 1 ? w->wf[1].p : nullptr};
   long long ws[2] = {w->wsize(0), cp->nblk > 1 ? w->wsize(1) : 0};
   if ((rc = launch_apply_table<T>(w->nw, cp->nblk, wfp, ws, w->logw.p, w->alive.p, cp->table.p, s))) return rc;
@@ -474,6 +447,7 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
   if (!rc) rc = cp->table.alloc(nw);
   if (!rc) rc = cp->comb_num.alloc(nw);
   if (!rc) rc = cp->qr_need.alloc(nw);
+  if (!rc) rc = cp->qr_flag.alloc(nw);
   if (!rc) rc = cp->comb_prob.alloc(nw);
   if (!rc) rc = cp->weight.alloc(nw);
   if (!rc) rc = cp->per_walker.alloc((size_t)nw * 8);
@@ -639,6 +613,89 @@ int afqmc_walkers_download(afqmc_walkers w, double* wf, double* logw, int* alive
   return 0;
 }
 
+int afqmc_walkers_download_some(afqmc_walkers w, int n, const int* index, double* wf, double* logw, int* alive) {
+  NOT_CAPTURING();
+  REQUIRE(w && (index || n == 0) && n >= 0, "null argument");
+  for (int k = 0; k < n; ++k)
+    REQUIRE(index[k] >= 0 && index[k] < w->nw, "walker index %d out of range (%d walkers)", index[k], w->nw);
+  afqmc_cp cp = w->cp;
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  const int L2 = 2 * cp->L, N = cp->N;
+  const long long per = (long long)L2 * N;
+  if (wf) {
+    std::fill(wf, wf + 2 * per * n, 0.0);
+    for (int b = 0; b < cp->nblk; ++b) {
+      const long long ws = w->wsize(b);
+      std::vector<char> raw(cp->esz() * ws);
+      for (int k = 0; k < n; ++k) {
+        HIP_TRY(hipMemcpy(raw.data(), w->wf[b].p + cp->esz() * ws * index[k], raw.size(), hipMemcpyDeviceToHost));
+        for (int j = 0; j < cp->Nb[b]; ++j)
+          for (int i = 0; i < cp->Mb[b]; ++i) {
+            const long long e = i + (long long)j * cp->Mb[b];
+            double re, im;
+            if (cp->fp32) {
+              const zf v = reinterpret_cast<const zf*>(raw.data())[e];
+              re = v.re; im = v.im;
+            } else {
+              const zd v = reinterpret_cast<const zd*>(raw.data())[e];
+              re = v.re; im = v.im;
+            }
+            double* dst = wf + 2 * (k * per + (cp->row0[b] + i) + (long long)(cp->col0[b] + j) * L2);
+            dst[0] = re;
+            dst[1] = im;
+          }
+      }
+    }
+  }
+  if (logw || alive) {
+    std::vector<zd> lw(w->nw);
+    std::vector<int> al(w->nw);
+    HIP_TRY(hipMemcpy(lw.data(), w->logw.p, sizeof(zd) * w->nw, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(al.data(), w->alive.p, sizeof(int) * w->nw, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) {
+      if (logw) { logw[2 * k] = lw[index[k]].re; logw[2 * k + 1] = lw[index[k]].im; }
+      if (alive) alive[k] = al[index[k]];
+    }
+  }
+  return 0;
+}
+
+int afqmc_walkers_set_alive(afqmc_walkers w, const int* alive) {
+  NOT_CAPTURING();
+  REQUIRE(w && alive, "null argument");
+  ScopedDevice sd(w->cp->ctx);
+  HIP_TRY(hipStreamSynchronize(w->cp->ctx->stream));
+  std::vector<int> a(alive, alive + w->nw);
+  for (int& x : a) x = x ? 1 : 0;
+  HIP_TRY(hipMemcpy(w->alive.p, a.data(), sizeof(int) * w->nw, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int afqmc_cp_walkers_from_trial(afqmc_cp cp, afqmc_walkers w, const double* logw) {
+  NOT_CAPTURING();
+  REQUIRE(cp && w && w->cp == cp, "invalid argument");
+  ScopedDevice sd(cp->ctx);
+  hipStream_t s = cp->ctx->stream;
+  HIP_TRY(hipStreamSynchronize(s));
+  // phiT into walker 0, then doubling device copies [0, k) -> [k, 2k)
+  for (int b = 0; b < cp->nblk; ++b) {
+    const size_t bytes = cp->esz() * (size_t)w->wsize(b);
+    HIP_TRY(hipMemcpyAsync(w->wf[b].p, cp->phiT[b].p, bytes, hipMemcpyDeviceToDevice, s));
+    for (int k = 1; k < w->nw; k *= 2) {
+      const int m = std::min(k, w->nw - k);
+      HIP_TRY(hipMemcpyAsync(w->wf[b].p + bytes * k, w->wf[b].p, bytes * m, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  const zd lw = cmk<double>(logw ? logw[0] : 0.0, logw ? logw[1] : 0.0);
+  std::vector<zd> h(w->nw, lw);
+  std::vector<int> a(w->nw, 1);
+  HIP_TRY(hipMemcpyAsync(w->logw.p, h.data(), sizeof(zd) * w->nw, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(w->alive.p, a.data(), sizeof(int) * w->nw, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
 int afqmc_cp_set_seed(afqmc_cp cp, uint64_t seed, uint64_t counter) {
   NOT_CAPTURING();
   REQUIRE(cp, "null argument");
@@ -672,6 +729,18 @@ int afqmc_cp_get_walker_steps(afqmc_cp cp, uint64_t* walker_steps) {
   DevScalars h;
   HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
   *walker_steps = h.walker_steps;
+  return 0;
+}
+
+int afqmc_cp_get_qr_stats(afqmc_cp cp, uint64_t* shifted, uint64_t* killed) {
+  NOT_CAPTURING();
+  REQUIRE(cp && shifted && killed, "null argument");
+  ScopedDevice sd(cp->ctx);
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  DevScalars h;
+  HIP_TRY(hipMemcpy(&h, cp->sc.p, sizeof(h), hipMemcpyDeviceToHost));
+  *shifted = h.qr_shifted;
+  *killed = h.qr_killed;
   return 0;
 }
 
@@ -918,6 +987,47 @@ int afqmc_cp_pop_control(afqmc_cp cp, afqmc_walkers w, double* weights_out, int*
   REQUIRE(!capturing() || (!weights_out && !table_out && !stats), "a captured comb cannot return outputs");
   ScopedDevice sd(cp->ctx);
   return DISPATCH(cp, pop_control_impl, cp, w, weights_out, table_out, stats);
+}
+
+int afqmc_cp_step_fields(afqmc_cp cp, afqmc_walkers w, const int* aux) {
+  NOT_CAPTURING();
+  REQUIRE(cp && w && w->cp == cp && aux, "invalid argument");
+  const long long n = (long long)w->nw * cp->L;
+  ScopedDevice sd(cp->ctx);
+  hipStream_t s = cp->ctx->stream;
+  std::vector<int> alive(w->nw);
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipMemcpy(alive.data(), w->alive.p, sizeof(int) * w->nw, hipMemcpyDeviceToHost));
+  for (int iw = 0; iw < w->nw; ++iw) {
+    if (!alive[iw]) continue;
+    for (int i = 0; i < cp->L; ++i) {
+      const int x = aux[(long long)iw * cp->L + i];
+      REQUIRE(x == 1 || x == -1, "field of walker %d, site %d is %d, not +1 or -1", iw, i, x);
+    }
+  }
+  if (cp->aux_in.n < (size_t)n) {
+    int rc = cp->aux_in.alloc((size_t)n);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(cp->aux_in.p, aux, sizeof(int) * n, hipMemcpyHostToDevice, s));
+  int rc = DISPATCH(cp, step_impl, cp, w, nullptr, nullptr, nullptr, cp->aux_in.p);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int afqmc_cp_pop_control_table(afqmc_cp cp, afqmc_walkers w, const int* table, double* weights_out) {
+  NOT_CAPTURING();
+  REQUIRE(cp && w && w->cp == cp && table, "invalid argument");
+  for (int i = 0; i < w->nw; ++i) {
+    REQUIRE(table[i] >= 0 && table[i] < w->nw, "table entry %d out of range", i);
+    REQUIRE(table[table[i]] == table[i], "table entry %d: parent %d does not keep its own slot", i, table[i]);
+  }
+  ScopedDevice sd(cp->ctx);
+  int rc = DISPATCH(cp, pop_control_impl, cp, w, weights_out, nullptr, nullptr, table);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(cp->ctx->stream));
+  return 0;
 }
 
 int afqmc_cp_measure(afqmc_cp cp, afqmc_walkers w) {

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <vector>
 
+#include "cholqr.hpp"
 #include "../../include/afqmc_mi355x.h"
 #include "cp_kernels.hpp"
 #include "gemm.hpp"
@@ -58,7 +59,7 @@ struct afqmc_ph_s {
   // workspace
   DevBuf<char> ovl[2], X[2], bgbuf, xs, Msamp, Tk[2], Dbuf, qrS, qr64, qr64b;
   DevBuf<zd> ldet[2], logO, den, tkin, ex, per_walker, bg_acc, ex_acc, frec, auxrec, slw, qrld;
-  DevBuf<int> info[2], prefix, table, comb_num, active, orders, nactive, qrinfo;
+  DevBuf<int> info[2], prefix, table, comb_num, active, orders, nactive, qrinfo, qr_need, qr_flag;
   DevBuf<double> weight, comb_prob;
   int dchunk = 1;
   size_t esz() const { return fp32 ? sizeof(zf) : sizeof(zd); }
@@ -274,51 +275,28 @@ int ph_step_impl(afqmc_ph ph, afqmc_ph_walkers w, double* force_out, double* aux
   return 0;
 }
 
-// CholeskyQR2 in FP64 on the L x Nmax matrix; logw += sum log R_kk + sum_{k<Nmin} log R_kk (SD2is.cpp:48-61)
-int ph_qr_f64(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, DevBuf<char>& tmp) {
-  hipStream_t s = ph->ctx->stream;
-  const int M = ph->L, N = ph->Nmax;
-  int rc;
-  for (int pass = 0; pass < 2; ++pass) {
-    GemmArgs a = gbase();
-    a.M = N; a.N = N; a.K = M; a.batch = w->nw;
-    a.A = wf.p; a.lda = M; a.strideA = w->wsize();
-    a.B = wf.p; a.ldb = M; a.strideB = w->wsize();
-    a.C = ph->qrS.p; a.ldc = N; a.strideC = (long long)N * N;
-    a.alive = w->alive.p;
-    a.herm = 1;
-    // measured fastest for the Gram shapes: one 128x128 tile per matrix above 64 columns
-    a.tile = N > 64 ? GEMM_TILE_128x128 : GEMM_TILE_64x64;
-    if ((rc = launch_cgemm<double>(OP_C, EPI_STORE, a, s))) return rc;
-    SmallDenseArgs c;
-    memset(&c, 0, sizeof(c));
-    c.N = N; c.batch = w->nw; c.in = ph->qrS.p; c.ld_in = N; c.stride_in = (long long)N * N;
-    c.out = ph->qrS.p; c.ld_out = N; c.stride_out = c.stride_in;
-    c.logdet = ph->qrld.p; c.info = ph->qrinfo.p; c.nsub = ph->Nmin; c.alive = w->alive.p; c.alive_div = 1;
-    if ((rc = launch_chol_inverse<double>(c, s))) return rc;
-    GemmArgs g = gbase();
-    g.M = M; g.N = N; g.K = N; g.batch = w->nw;
-    g.A = wf.p; g.lda = M; g.strideA = w->wsize();
-    g.B = ph->qrS.p; g.ldb = N; g.strideB = (long long)N * N;
-    g.C = tmp.p; g.ldc = M; g.strideC = w->wsize();
-    g.alive = w->alive.p;
-    g.b_upper = 1;  // R^{-1} is upper triangular
-    g.tile = GEMM_TILE_32x32;  // measured fastest for W R^{-1}
-    if ((rc = launch_cgemm<double>(OP_N, EPI_STORE, g, s))) return rc;
-    std::swap(wf, tmp);
-    if ((rc = launch_logw_add_real(w->nw, w->logw.p, w->alive.p, ph->qrld.p, nullptr, 0, 1.0, 1.0, s))) return rc;
-  }
-  return 0;
+// SD2is::stabilize (SD2is.cpp:48-61) by robust CholeskyQR in FP64 (cholqr.hpp) on the L x Nmax matrix:
+// logw += sum log R_kk + sum_{k<Nmin} log R_kk; tol: second pass where |Q^H Q - I| exceeds it
+int ph_qr_f64(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, DevBuf<char>& tmp, double tol) {
+  CholQrBlock q;
+  q.nw = w->nw; q.M = ph->L; q.N = ph->Nmax; q.nsub = ph->Nmin;
+  q.wf = &wf; q.tmp = &tmp;
+  q.S = reinterpret_cast<zd*>(ph->qrS.p); q.ld = ph->qrld.p;
+  q.info = ph->qrinfo.p; q.need = ph->qr_need.p; q.flag = ph->qr_flag.p;
+  q.alive = w->alive.p; q.logw = w->logw.p;
+  q.m0 = 1.0; q.m1 = 1.0; q.tol = tol; q.sc = ph->sc.p;
+  return cholqr_f64(q, ph->ctx->stream);
 }
 
 int ph_stabilize_impl(afqmc_ph ph, afqmc_ph_walkers w) {
   RegionTimer t(ph->ctx, REG_QR);
-  if (!ph->fp32) return ph_qr_f64(ph, w, w->wf, w->tmp);
+  if (!ph->fp32) return ph_qr_f64(ph, w, w->wf, w->tmp, 1e-13);
   hipStream_t s = ph->ctx->stream;
   const long long n = w->wsize() * w->nw;
   int rc;
   if ((rc = launch_convert_f32_f64(w->wf.p, ph->qr64.p, n, s))) return rc;
-  if ((rc = ph_qr_f64(ph, w, ph->qr64, ph->qr64b))) return rc;
+  // the result is rounded to FP32, so a first-pass Q orthonormal to 1e-8 is final
+  if ((rc = ph_qr_f64(ph, w, ph->qr64, ph->qr64b, 1e-8))) return rc;
   return launch_convert_f64_f32(ph->qr64.p, w->wf.p, n, s);
 }
 
@@ -519,6 +497,8 @@ int afqmc_ph_create(afqmc_ctx ctx, const afqmc_ph_desc* d, afqmc_ph* out) {
   if (!rc) rc = ph->qrS.alloc_bytes(sizeof(zd) * (size_t)ph->Nmax * ph->Nmax * nw);
   if (!rc) rc = ph->qrld.alloc(nw);
   if (!rc) rc = ph->qrinfo.alloc(nw);
+  if (!rc) rc = ph->qr_need.alloc(nw);
+  if (!rc) rc = ph->qr_flag.alloc(nw);
   if (!rc) rc = ph->qr64.alloc_bytes(sizeof(zd) * wsz * nw);
   if (!rc) rc = ph->qr64b.alloc_bytes(sizeof(zd) * wsz * nw);
   if (!rc) rc = ph->logO.alloc(nw);
@@ -692,6 +672,17 @@ int afqmc_ph_get_counter(afqmc_ph ph, uint64_t* counter, uint64_t* walker_steps)
   *counter = h.counter;
   if (walker_steps) *walker_steps = h.walker_steps;
   REQUIRE(!h.uniform_overrun, "the host-provided uniforms were exhausted (%llu given)", h.ulen);
+  return 0;
+}
+
+int afqmc_ph_get_qr_stats(afqmc_ph ph, uint64_t* shifted, uint64_t* killed) {
+  REQUIRE(ph && shifted && killed, "null argument");
+  ScopedDevice sdv(ph->ctx);
+  DevScalars h;
+  int rc = ph_scalars(ph, &h, false);
+  if (rc) return rc;
+  *shifted = h.qr_shifted;
+  *killed = h.qr_killed;
   return 0;
 }
 

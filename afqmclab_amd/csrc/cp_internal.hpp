@@ -34,9 +34,9 @@ struct afqmc_cp_s {
   // workspace
   DevBuf<char> ovl[2], scale[2], d2x2, qrS[2], qr64[2], qr64b[2];
   DevBuf<zd> ldet[2], qrld[2], logO, gpart[3][2], per_walker, site_acc, site_part, den;
-  DevBuf<int> info[2], qrinfo[2], prefix, table, comb_num, qr_need;
+  DevBuf<int> info[2], qrinfo[2], prefix, table, comb_num, qr_need, qr_flag;
   DevBuf<double> weight, comb_prob, force_rec;
-  DevBuf<int> aux_rec;
+  DevBuf<int> aux_rec, aux_in;
   int ntile[2] = {1, 1};
   int tile[3] = {GEMM_TILE_64x64, GEMM_TILE_64x64, GEMM_TILE_64x64};  // propagate, overlap, green
   std::vector<cd> h_gamma, h_cdiag;

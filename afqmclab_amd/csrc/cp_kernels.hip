@@ -216,11 +216,16 @@ __global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
       if (f > a.cap) f = a.cap;
       if (f < -a.cap) f = -a.cap;
     }
-    const unsigned long long k = base + (unsigned long long)i;
-    const double u = afqmc_take_uniform(a.ubuf, a.sc, key, k);
     const double em = exp(-f), ep = exp(f);
-    const double prob = em / (em + ep);
-    const int x = (u < prob) ? -1 : 1;
+    int x;
+    if (a.aux_in) {
+      x = a.aux_in[(long long)w * L + i];
+    } else {
+      const unsigned long long k = base + (unsigned long long)i;
+      const double u = afqmc_take_uniform(a.ubuf, a.sc, key, k);
+      const double prob = em / (em + ep);
+      x = (u < prob) ? -1 : 1;
+    }
     lp += x * f - log(ep + em);
     const zd gx = cscale(a.gamma[i], (double)x);
     xg = cadd(xg, gx);
@@ -551,6 +556,46 @@ __global__ void ortho_check_kernel(int N, const zd* S, const int* alive, double 
   }
 }
 
+__global__ void qr_flag_kernel(int nw, const int* alive, const int* info, int* flag) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nw) flag[w] = (alive[w] && info[w]) ? 1 : 0;
+}
+
+// S_w += c tr(S_w) I for flagged walkers: one workgroup per walker
+__global__ void qr_shift_kernel(int N, zd* S, const int* flag, double c) {
+  const int w = blockIdx.x;
+  if (!flag[w]) return;
+  zd* s = S + (long long)w * N * N;
+  __shared__ double sh[4];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) t += fabs(s[i + (long long)i * N].re);
+  t = block_sum_d(t, sh);
+  const double shift = c * t;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) s[i + (long long)i * N].re += shift;
+}
+
+// one workgroup: flagged walkers count as shifted, those whose shifted Cholesky broke down as well
+// die (their Gram matrix is singular: a rank-deficient walker); the counts are added by one thread
+__global__ void qr_kill_kernel(int nw, int* flag, const int* info, int* alive, DevScalars* sc, int count_shifted) {
+  __shared__ double sh[16];
+  double nshift = 0.0, nkill = 0.0;
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+    if (!flag[w]) continue;
+    nshift += 1.0;
+    if (info[w]) {
+      alive[w] = 0;
+      flag[w] = 0;
+      nkill += 1.0;
+    }
+  }
+  nshift = block_sum_d(nshift, sh);
+  nkill = block_sum_d(nkill, sh);
+  if (threadIdx.x == 0) {
+    if (count_shifted) sc->qr_shifted += (unsigned long long)nshift;
+    sc->qr_killed += (unsigned long long)nkill;
+  }
+}
+
 __global__ void set_et_kernel(DevScalars* sc) {
   zd e = cdiv(sc->HNum, sc->den);
   sc->ET = e.re;
@@ -841,6 +886,28 @@ int launch_energy(const EnergyArgs& a, hipStream_t s) {
 int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, hipStream_t s) {
   if (nw <= 0) return 0;
   hipLaunchKernelGGL(ortho_check_kernel, dim3(nw), dim3(256), 0, s, N, S, alive, tol, need);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_qr_flag(int nw, const int* alive, const int* info, int* flag, hipStream_t s) {
+  if (nw <= 0) return 0;
+  hipLaunchKernelGGL(qr_flag_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, nw, alive, info, flag);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_qr_shift(int nw, int N, zd* S, const int* flag, double c, hipStream_t s) {
+  if (nw <= 0) return 0;
+  hipLaunchKernelGGL(qr_shift_kernel, dim3(nw), dim3(64), 0, s, N, S, flag, c);
+  HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_qr_kill(int nw, int* flag, const int* info, int* alive, DevScalars* sc, int count_shifted,
+                   hipStream_t s) {
+  if (nw <= 0) return 0;
+  hipLaunchKernelGGL(qr_kill_kernel, dim3(1), dim3(1024), 0, s, nw, flag, info, alive, sc, count_shifted);
   HIP_LAUNCH_CHECK();
   return 0;
 }

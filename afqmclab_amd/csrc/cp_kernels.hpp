@@ -18,6 +18,8 @@ struct DevScalars {
   double taylor_orders;        // phaseless: total Taylor terms applied in the last step
   unsigned long long walker_steps;  // walkers propagated, summed over steps (alive walkers only)
   unsigned long long ulen;     // number of host-provided uniforms (when a uniform buffer is set)
+  unsigned long long qr_shifted;  // walkers whose Gram matrix needed the shifted Cholesky pass
+  unsigned long long qr_killed;   // walkers whose Gram matrix stayed singular (rank deficient)
 };
 
 __device__ __forceinline__ double afqmc_uniform_dev(unsigned long long key, unsigned long long k) {
@@ -76,6 +78,7 @@ struct HsSampleArgs {
   void* scale[2]; int diag_only;     // complex T, [nw][Mb] per block
   void* d2x2;                        // complex T, [nw][4][L] when !diag_only
   double* force_out; int* aux_out;   // optional records
+  const int* aux_in;                 // optional given fields [nw][L] (no uniforms drawn)
   int fp32;
 };
 int launch_hs_sample(const HsSampleArgs& a, hipStream_t s);
@@ -107,6 +110,14 @@ int launch_energy(const EnergyArgs& a, hipStream_t s);
 int launch_set_et_from_energy(DevScalars* sc, hipStream_t s);
 // CholeskyQR: which walkers still need a second pass (S = Q^H Q farther than tol from I)
 int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, hipStream_t s);
+// flag[w] = alive[w] && info[w] (a Cholesky breakdown of walker w)
+int launch_qr_flag(int nw, const int* alive, const int* info, int* flag, hipStream_t s);
+// S_w += c * tr(S_w) * I for the flagged walkers (the shift of shifted CholeskyQR)
+int launch_qr_shift(int nw, int N, zd* S, const int* flag, double c, hipStream_t s);
+// flagged walkers whose Cholesky broke down (info) die and leave the flag; counts into sc (the flagged
+// ones as shifted when count_shifted)
+int launch_qr_kill(int nw, int* flag, const int* info, int* alive, DevScalars* sc, int count_shifted,
+                   hipStream_t s);
 int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int use_imag_sub,
                          double mult0, double mult1, hipStream_t s);
 

@@ -432,7 +432,7 @@ __global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
     if constexpr (G2) {
       if (kr == 2) skk = readfirstlane_d(g2[wv][CK][kl].re);
     }
-    const bool ok = skk > 0.0;
+    const bool ok = skk > p.pivot_tol;
     if (!ok) skk = 1.0;
 #pragma unroll
     for (int r = 0; r < RR; ++r) {

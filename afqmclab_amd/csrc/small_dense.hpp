@@ -11,6 +11,7 @@ struct SmallDenseArgs {
   int nsub;            // chol only
   const int* alive;    // optional: skip matrix b if alive[b / alive_div] == 0
   int alive_div;
+  double pivot_tol;    // chol only: info = 1 also when a pivot of the Jacobi-scaled matrix is <= this
 };
 
 template <typename T> int launch_gj_inverse(const SmallDenseArgs& a, hipStream_t s);

@@ -278,3 +278,23 @@ def ghf_trial(model: HubbardModel, density, spin_orbit, converge_type="energy", 
         raise ScfNotConverged(f"GHF did not converge in {max_iter} iterations ({converge_type} change {conv:.3e})")
     return np.asfortranarray(state), {"iterations": it, "energy": energy, "density": rho_new,
                                       "spin_orbit": so_new, "converge_value": conv}
+
+
+def heterogeneous_hubbard_instances(n_each=6, seed=20241, sizes=(6, 8, 10, 12), fills=(0.75, 0.875, 1.0)):
+    """A batch of independent Hubbard lattices: n_each periodic n x n lattices for each n in sizes,
+    filling fills[j % 3] with the electron count rounded to the nearest even number (Nup = Ndn),
+    U uniform in [2, 8] (seeded) and t' = -0.2 on every odd instance.  Returns a list of
+    (name, model, closed-shell spin-block trial, Nup)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in sizes:
+        for j in range(n_each):
+            idx = len(out)
+            Ls = n * n
+            ntot = int(round(fills[j % len(fills)] * Ls / 2.0)) * 2
+            nup = ntot // 2
+            U = float(rng.uniform(2.0, 8.0))
+            t2 = -0.2 if idx % 2 == 1 else 0.0
+            model = HubbardModel.square(n, n, ntot, U, 1.0, t2)
+            out.append((f"{n}x{n} N={ntot} U={U:.2f} t'={t2}", model, spin_block_trial(model, nup, nup), nup))
+    return out

@@ -258,6 +258,12 @@ class PhaselessPropagator:
     def onebody(self, w, which=0):
         check(lib().afqmc_ph_onebody(self.h, w.h, int(which)))
 
+    def qr_stats(self):
+        """(walkers redone by shifted CholeskyQR, walkers marked dead as rank deficient)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().afqmc_ph_get_qr_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def stabilize(self, w):
         check(lib().afqmc_ph_stabilize(self.h, w.h))
 

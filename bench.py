@@ -61,7 +61,23 @@ def flops_per_walker_step(Ls, nup, ndn, mgs, pop, meas):
     return 8.0 * (step + qr + comb + measure), 8.0 * sum(Ls * Ls * n for n in (nup, ndn))
 
 
-def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, keep_walkers=False, block=50):
+def timed_window(thermal, warmup, steps, pop):
+    """First step of the timed window: after the thermal (ET-adjust) phase and `warmup` more steps,
+    moved on (never back) so that the window holds a comb when it is shorter than the comb period
+    (a comb follows step i when (i + 1) % pop == 0)."""
+    s0 = thermal + warmup
+    if steps < pop:
+        comb = s0 + (-(s0 + 1)) % pop  # first comb step at or after s0
+        if comb >= s0 + steps:
+            s0 = comb - steps // 2
+    return s0
+
+
+def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, keep_walkers=False):
+    """The CP loop of afqmcConstraintPath.cpp:99-150 on the device: QR every mgs steps, comb every pop
+    steps, ET adjusted every et_step steps while i < et_max, the mixed estimator every meas steps
+    after the thermal phase.  The thermal phase, `warmup` steps and the alignment of the window
+    run untimed; the timed window is `steps` steps at the full cadence."""
     from afqmclab_amd.cp import CPMethod, CPPropagator
     m = CPMethod(dt=sched["dt"], walkerSizePerThread=nw, decompType="densitySpin", forceType="dynamicForce",
                  forceCap=1.5, mgsStep=sched["mgs"], popControlStep=sched["pop"], ET=-10.0)
@@ -69,7 +85,7 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, ke
     prop.set_seed(seed, 0)
     prop.set_ET(m.ET)
     ws = prop.walkers(nw)
-    ws.set_from_trial(phi)
+    ws.set_from_trial()
     # initialWalker + estimateMemory + projectExpMinusHalfDtK (afqmcConstraintPath.cpp:14-30, :108)
     prop.stabilize(ws)
     prop.pop_control(ws)
@@ -80,7 +96,7 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, ke
     tiles = [CPPropagator.TILES[t] for t in tiles]
     ctx.synchronize()
     idx = [0, 0]
-    energies = []
+    counts = {"measure": 0, "qr": 0, "comb": 0}
 
     def one_step(i):
         if i < sched["thermal"]:
@@ -89,33 +105,42 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, ke
                 prop.adjust_ET()
         elif (i - sched["thermal"]) % sched["meas"] == 0:
             prop.measure(ws)
+            counts["measure"] += 1
         prop.step(ws)
         idx[0] += 1
         if idx[0] == sched["mgs"]:
             prop.stabilize(ws)
             idx[0] = 0
+            counts["qr"] += 1
         idx[1] += 1
         if idx[1] == sched["pop"]:
             prop.pop_control(ws)
             idx[1] = 0
+            counts["comb"] += 1
 
-    for i in range(warmup):
+    s0 = timed_window(sched["thermal"], warmup, steps, sched["pop"])
+    for i in range(s0):
         one_step(i)
     ctx.synchronize()
     prop.reset_measure()
     ws0 = prop.walker_steps()
+    for k in counts:
+        counts[k] = 0
+    # energies read after every `block` steps (a multiple of the measurement period)
+    block = max(sched["meas"], min(50, (steps // 4) // sched["meas"] * sched["meas"]))
     ctx.set_timing(True)
     ctx.synchronize()
-    blocks = []  # (HNum, den) of consecutive measurement blocks after thermalisation
+    blocks = []  # (HNum, den) of consecutive blocks of the window
     t0 = time.perf_counter()
-    for i in range(warmup, warmup + steps):
+    for i in range(s0, s0 + steps):
         one_step(i)
-        if i >= sched["thermal"] and (i - sched["thermal"] + 1) % block == 0:
+        if (i - s0 + 1) % block == 0:
             e = prop.energy()
-            blocks.append((e["HNum"], e["den"]))
+            if abs(e["den"]) > 0:
+                blocks.append((e["HNum"], e["den"]))
             prop.reset_measure()
-        if (i - warmup + 1) % 100 == 0:
-            log(f"[bench]   step {i - warmup + 1}/{steps}")
+        if (i - s0 + 1) % 100 == 0:
+            log(f"[bench]   step {i - s0 + 1}/{steps}")
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     timing = ctx.timing()
@@ -133,7 +158,73 @@ def run_path(ctx, model, phi, nup, precision, nw, steps, warmup, seed, sched, ke
     ws.close()
     prop.close()
     return dict(elapsed=elapsed, walker_steps=wsteps, timing=timing, energy=E, err=err, nblocks=len(eb),
-                walkers=walkers, tiles=tiles)
+                walkers=walkers, tiles=tiles, window=(s0, s0 + steps), counts=counts)
+
+
+def precision_replay(ctx, model, phi, nup, nw, sched, seed, thermal, nsteps, tiles=None):
+    """The precision error of the mixed path's energy, free of sampling noise: an FP64 run draws its
+    fields and comb tables, and a mixed-precision run from the same initial walkers replays them
+    step by step (afqmc_cp_step_fields, afqmc_cp_pop_control_table) with the FP64 run's ET.
+    `thermal` steps with ET adjustment, then `nsteps` steps with the mixed estimator every meas
+    steps, QR every mgs and a comb every pop steps in both.  Returns both energies and the
+    differences per site."""
+    from afqmclab_amd.cp import CPMethod, CPPropagator
+    props, sets = [], []
+    for precision in (0, 1):
+        m = CPMethod(dt=sched["dt"], walkerSizePerThread=nw, decompType="densitySpin", forceType="dynamicForce",
+                     forceCap=1.5, mgsStep=sched["mgs"], popControlStep=sched["pop"], ET=-10.0)
+        prop = CPPropagator(ctx, model, m, phi, nup=nup, spin_blocks=True, precision=precision, max_walkers=nw)
+        if tiles and precision == 1:
+            prop.set_tiles(*[CPPropagator.TILES.index(t) for t in tiles])
+        prop.set_ET(m.ET)
+        ws = prop.walkers(nw)
+        ws.set_from_trial()
+        prop.stabilize(ws)
+        prop.onebody(ws, 1)
+        props.append(prop)
+        sets.append(ws)
+    p64, p32 = props
+    w64, w32 = sets
+    p64.set_seed(seed, 0)
+    e64, e32 = [], []
+    for i in range(thermal + nsteps):
+        measured = False
+        if i < thermal:
+            if i % sched["et_step"] == 0:
+                p64.measure(w64)
+                p64.adjust_ET()
+                p32.set_ET(p64.get_ET())
+        elif (i - thermal) % sched["meas"] == 0:
+            for p, w in ((p64, w64), (p32, w32)):
+                p.reset_measure()
+                p.measure(w)
+            measured = True
+        _, aux = p64.step(w64, record=True)
+        # a walker the FP64 step killed (its row holds no fields) keeps +1 fields in the mixed run
+        p32.step_fields(w32, np.where(w64.alive()[:, None] != 0, aux, 1))
+        if (i + 1) % sched["mgs"] == 0:
+            p64.stabilize(w64)
+            p32.stabilize(w32)
+        if (i + 1) % sched["pop"] == 0:
+            _, table, _ = p64.pop_control(w64, record=True)
+            p32.pop_control_table(w32, table)
+        if measured:
+            a, b = p64.energy(), p32.energy()
+            e64.append((a["HNum"], a["den"]))
+            e32.append((b["HNum"], b["den"]))
+    for w in sets:
+        w.close()
+    for p in props:
+        p.close()
+    Ls = model.L
+    E64 = sum(h for h, d in e64) / sum(d for h, d in e64)
+    E32 = sum(h for h, d in e32) / sum(d for h, d in e32)
+    per = [abs((a[0] / a[1] - b[0] / b[1]) / Ls) for a, b in zip(e64, e32)]
+    return {"workload": f"same walkers, fields and comb tables in both paths: {thermal} steps with ET "
+                        f"adjustment, then {nsteps} steps with {len(e64)} measurements",
+            "E_per_site_fp64": float((E64 / Ls).real), "E_per_site_mixed": float((E32 / Ls).real),
+            "dE_per_site": float(abs((E32 - E64) / Ls)), "max_dE_per_site_one_measurement": float(max(per)),
+            "measurements": len(e64)}
 
 
 def estimator_check(ctx, model, phi, nup, nw, walkers):
@@ -198,21 +289,8 @@ def run_phaseless(ctx, mol, precision, nw, steps, warmup, seed, meas=5):
 def hetero_instances(n_each=6, seed=20241):
     """24 Hubbard instances: 6x6, 8x8, 10x10, 12x12 (n_each each), filling 0.75 / 0.875 / 1.0 with
     N rounded to the nearest even number (Nup = Ndn), U uniform in [2, 8], t' = -0.2 on odd ones."""
-    from afqmclab_amd.model import HubbardModel, spin_block_trial
-    rng = np.random.default_rng(seed)
-    out = []
-    fills = [0.75, 0.875, 1.0]
-    for li, n in enumerate([6, 8, 10, 12]):
-        for j in range(n_each):
-            idx = len(out)
-            Ls = n * n
-            ntot = int(round(fills[j % 3] * Ls / 2.0)) * 2
-            nup = ntot // 2
-            U = float(rng.uniform(2.0, 8.0))
-            t2 = -0.2 if idx % 2 == 1 else 0.0
-            model = HubbardModel.square(n, n, ntot, U, 1.0, t2)
-            out.append((f"{n}x{n} N={ntot} U={U:.2f} t'={t2}", model, spin_block_trial(model, nup, nup), nup))
-    return out
+    from afqmclab_amd.model import heterogeneous_hubbard_instances
+    return heterogeneous_hubbard_instances(n_each, seed)
 
 
 def run_hetero(instances, nw, steps, seed, tuned_graph):
@@ -395,18 +473,27 @@ def run_hs_bandwidth(ctx, nw, steps=6):
 
 
 def cpu_worker(argv):
-    """One walker stream of the CPU baseline (run as `bench.py --cpu-worker DIR RANK WALKERS SECONDS`):
-    the oracle's CP step on dense 2Ls x N walkers with single-threaded BLAS, timed after set-up."""
-    d, rank, nw, budget = argv[0], int(argv[1]), int(argv[2]), float(argv[3])
+    """One walker stream of the CPU baseline (run as
+    `bench.py --cpu-worker DIR RANK WALKERS SECONDS REPRESENTATION`): the oracle's CP loop with
+    single-threaded BLAS on dense 2Ls x N walkers (the reference's representation) or on spin blocks
+    (the representation the device executes), QR every mgs, comb every pop and the mixed estimator
+    every meas steps, timed after set-up."""
+    d, rank, nw, budget, rep = argv[0], int(argv[1]), int(argv[2]), float(argv[3]), argv[4]
     from oracle.blocks import SD
     from oracle.drivers import AfqmcConstraintPath as OracleCP, CPMethod as OMethod
     from oracle.models import HubbardSOC
     from oracle.rng import UniformStream
+    from oracle.spin_blocks import AfqmcConstraintPathSpinBlocks
     z = np.load(os.path.join(d, "inputs.npz"), allow_pickle=False)
     om = HubbardSOC(int(z["L"]), int(z["N"]), z["K"], U=z["U"])
     m = OMethod(dt=float(z["dt"]), walkerSizePerThread=nw, decompType="densitySpin", mgsStep=int(z["mgs"]),
                 popControlStep=int(z["pop"]), thermalSize=0, ETAdjustMaxSize=0, ET=-10.0)
-    drv = OracleCP(m, om, UniformStream(int(z["seed"]) + rank), phiT=SD(wf=z["phi"]))
+    rng = UniformStream(int(z["seed"]) + rank)
+    if rep == "blocks":
+        drv = AfqmcConstraintPathSpinBlocks(m, om, rng, SD(wf=z["phi"]), int(z["nup"]))
+    else:
+        drv = OracleCP(m, om, rng, phiT=SD(wf=z["phi"]))
+    meas = int(z["meas"])
     drv.initialWalker()
     drv.estimateMemory()
     drv.projectExpMinusHalfDtK()
@@ -414,6 +501,8 @@ def cpu_worker(argv):
     nsteps = 0
     t0 = time.perf_counter()
     while True:
+        if nsteps % meas == 0:
+            drv.addMixedMeasurement()
         drv.projectOneStep(idx)
         nsteps += 1
         if time.perf_counter() - t0 > budget:
@@ -422,20 +511,21 @@ def cpu_worker(argv):
     return 0
 
 
-def cpu_baseline(model, phi, nw_per, budget_s, sched, seed):
-    """The oracle (dense reference representation) on a bounded sample of the same workload, one
-    walker stream per host core with single-threaded BLAS (the reference runs one MPI rank per core):
-    as many worker processes as cores, each with nw_per walkers and its own comb, all timed
-    concurrently; the rate is the sum of the workers' walker-steps/s."""
+def cpu_baseline(model, phi, nup, nw_per, budget_s, sched, seed, rep):
+    """The oracle on a bounded sample of the same workload, one walker stream per host core with
+    single-threaded BLAS (the reference runs one MPI rank per core): as many worker processes as
+    cores, each with nw_per walkers and its own comb, all timed concurrently; the rate is the sum of
+    the workers' walker-steps/s.  rep: "dense" (2Ls x N walkers) or "blocks" (spin blocks)."""
     import subprocess
     import tempfile
     cores = len(os.sched_getaffinity(0))
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
     with tempfile.TemporaryDirectory() as d:
-        np.savez(os.path.join(d, "inputs.npz"), L=model.L, N=model.N, K=model.K, U=model.U, phi=phi,
-                 dt=sched["dt"], mgs=sched["mgs"], pop=sched["pop"], seed=seed)
+        np.savez(os.path.join(d, "inputs.npz"), L=model.L, N=model.N, K=model.K, U=model.U, phi=phi, nup=nup,
+                 dt=sched["dt"], mgs=sched["mgs"], pop=sched["pop"], meas=sched["meas"], seed=seed)
         procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", d, str(r), str(nw_per),
-                                   str(budget_s)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+                                   str(budget_s), rep], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                                  env=env)
                  for r in range(cores)]
         outs = []
         for pr in procs:
@@ -474,6 +564,9 @@ def main():
                     help="back-propagation leg: measurements with a 40-step window every 20 steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (reference kernel for HBM traffic counters)")
+    ap.add_argument("--replay-steps", type=int, default=100,
+                    help="precision leg: steps measured after 50 ET-adjust steps with the FP64 run's fields "
+                         "and tables replayed in the mixed path (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -503,11 +596,13 @@ def main():
         precision = {"fp64": 0, "mixed": 1, "split_bf16": 2}[path]
         log(f"[bench] path {path}: {args.walkers} walkers, {args.warmup} warmup + {args.steps} timed steps")
         results[path] = run_path(ctx, model, phi, nup, precision, args.walkers, args.steps, args.warmup, args.seed,
-                                 sched, keep_walkers=(path == "fp64"))
+                                 sched, keep_walkers=(path == "fp64" and args.walkers * Ls * nup <= 2048 * 256 * 112))
         r = results[path]
         log(f"[bench]   {path}: {r['elapsed']:.3f}s, {r['walker_steps'] / r['elapsed']:.1f} walker-steps/s, "
             f"E={r['energy']:.8f}")
-    main_path = "mixed" if "mixed" in results else list(results)[0]
+    if "mixed" not in results and "fp64" not in results:
+        raise SystemExit("--paths must include mixed or fp64 (the headline path)")
+    main_path = "mixed" if "mixed" in results else "fp64"
     r = results[main_path]
     value = r["walker_steps"] / r["elapsed"]
     fl_ws, fl_prop = flops_per_walker_step(Ls, nup, ndn, sched["mgs"], sched["pop"], sched["meas"])
@@ -534,7 +629,8 @@ def main():
         "config": {"workload": f"CP-AFQMC Hubbard {nx}x{ny} PBC U={args.U} Nup=Ndn={nup} UHF trial dt=0.005 "
                                f"{args.walkers} walkers, spin-block walkers 2x({Ls}x{nup})",
                    "walkers": args.walkers, "path": main_path, "qr_every": sched["mgs"], "comb_every": sched["pop"],
-                   "measure_every": sched["meas"]},
+                   "measure_every": sched["meas"], "thermal_untimed": sched["thermal"],
+                   "timed_window": list(r["window"]), "in_window": r["counts"]},
         "roofline": {"bound": "mfma", "kernel": f"cgemm propagate (exp(-dt K) diag(HS) W), {r['tiles'][0]} tiles",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": (achieved / peak) if achieved else None, "traffic": None,
@@ -569,8 +665,9 @@ def main():
         out["mixed_vs_fp64"] = (mx["walker_steps"] / mx["elapsed"]) / (f["walker_steps"] / f["elapsed"])
         out["dE_per_site_mixed_vs_fp64"] = abs((mx["energy"] - f["energy"]) / Ls)
         out["dE_per_site_stat_err"] = float(np.hypot(f["err"], mx["err"]) / Ls)
-        e64, e32 = estimator_check(ctx, model, phi, nup, args.walkers, f["walkers"])
-        out["dE_per_site_estimator_same_walkers"] = abs((e32 - e64) / Ls)
+        if f["walkers"] is not None:
+            e64, e32 = estimator_check(ctx, model, phi, nup, args.walkers, f["walkers"])
+            out["dE_per_site_estimator_same_walkers"] = abs((e32 - e64) / Ls)
     if args.ph_steps > 0:
         from afqmclab_amd.phaseless import synthetic_molecule
         t0 = time.time()
@@ -627,16 +724,35 @@ def main():
             f"{out['hubbard4x4_vs_exact']['err']:.4f}, exact {out['hubbard4x4_vs_exact']['E_exact']:.4f}")
         out["gradient_8x8"] = run_gradient_leg(ctx)
         log(f"[bench] 8x8 fixed-field gradient: {out['gradient_8x8']}")
+    if args.replay_steps > 0 and "mixed" in results:
+        log(f"[bench] precision replay: {args.walkers} walkers, 50 + {args.replay_steps} steps")
+        rep = precision_replay(ctx, model, phi, nup, args.walkers, sched, args.seed + 1, 50, args.replay_steps,
+                               results["mixed"]["tiles"])
+        out["precision_replay"] = rep
+        out["dE_per_site_replay"] = rep["dE_per_site"]
+        log(f"[bench]   replay: {rep}")
     if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle, one process per core, {args.cpu_walkers} walkers each, "
-            f"~{args.cpu_seconds}s")
-        ws_s, cores, nst = cpu_baseline(model, phi, args.cpu_walkers, args.cpu_seconds, sched, args.seed)
-        out["cpu_baseline"] = {"value": ws_s, "unit": "walker-steps/s", "cores": cores, "kind": "port",
-                               "sample": f"{cores} processes (one per core, single-threaded OpenBLAS) x "
-                                         f"{args.cpu_walkers} walkers, >= {nst} steps each in ~{args.cpu_seconds:.0f} s, "
-                                         f"dense {2 * Ls}x{2 * nup} walkers of the same workload (numpy/LAPACK "
-                                         f"oracle)"}
-        out["gpu_vs_cpu_fp64"] = out["paths"].get("fp64", out["paths"][main_path])["walker_steps_per_s"] / ws_s
+        cpu = {}
+        for rep in ("blocks", "dense"):
+            log(f"[bench] cpu baseline ({rep}): oracle, one process per core, {args.cpu_walkers} walkers each, "
+                f"~{args.cpu_seconds}s")
+            ws_s, cores, nst = cpu_baseline(model, phi, nup, args.cpu_walkers, args.cpu_seconds, sched, args.seed, rep)
+            cpu[rep] = {"value": ws_s, "min_steps_per_worker": nst}
+            log(f"[bench]   {rep}: {ws_s:.1f} walker-steps/s on {cores} cores")
+        shape = {"blocks": f"spin-block 2 x ({Ls}x{nup}) walkers", "dense": f"dense {2 * Ls}x{2 * nup} walkers"}
+        out["cpu_baseline"] = {
+            "value": cpu["blocks"]["value"], "unit": "walker-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} processes (one per core, single-threaded BLAS) x {args.cpu_walkers} walkers each, "
+                      f"~{args.cpu_seconds:.0f} s per representation, the numpy/LAPACK oracle's CP loop with QR every "
+                      f"{sched['mgs']}, comb every {sched['pop']} and the mixed estimator every {sched['meas']} steps; "
+                      f"value: {shape['blocks']} (the device's representation, >= {cpu['blocks']['min_steps_per_worker']} "
+                      f"steps per worker); dense: {shape['dense']} as the reference runs them",
+            "value_dense": cpu["dense"]["value"],
+        }
+        fp = out["paths"].get("fp64", out["paths"][main_path])["walker_steps_per_s"]
+        out["gpu_vs_cpu_fp64"] = fp / cpu["blocks"]["value"]
+        out["gpu_vs_cpu_fp64_same_representation"] = fp / cpu["blocks"]["value"]
+        out["gpu_vs_cpu_fp64_vs_dense_reference_representation"] = fp / cpu["dense"]["value"]
     print(json.dumps(out))
 
 

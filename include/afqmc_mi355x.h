@@ -105,6 +105,14 @@ int afqmc_walkers_count(afqmc_walkers w);
 /* wf: nw x (2L x N) complex, logw: nw complex, alive: nw ints (NULL = all alive) */
 int afqmc_walkers_upload(afqmc_walkers w, const double* wf, const double* logw, const int* alive);
 int afqmc_walkers_download(afqmc_walkers w, double* wf, double* logw, int* alive);
+/* SD::write of a few walkers (SD.cpp:85-103): the n walkers index[0..n-1], wf n x (2L x N), logw n complex,
+ * alive n ints (any output may be NULL) */
+int afqmc_walkers_download_some(afqmc_walkers w, int n, const int* index, double* wf, double* logw, int* alive);
+/* walkerIsAlive of the driver (afqmcConstraintPath.h:29): nw ints, nonzero = alive */
+int afqmc_walkers_set_alive(afqmc_walkers w, const int* alive);
+/* setWalkerFromPhiT / initialWalker (SDSDOperation.cpp:176-186, afqmcConstraintPathSetWalker.cpp:9-18):
+ * every walker = the trial held on the device, logw (1 complex, NULL = 0), all alive */
+int afqmc_cp_walkers_from_trial(afqmc_cp cp, afqmc_walkers w, const double* logw);
 
 /* random numbers: a counter-based uniform stream on the device, or host-provided
  * uniforms in the reference's consumption order (uniformHao, random_hao.cpp:73-76) */
@@ -143,8 +151,15 @@ int afqmc_cp_onebody(afqmc_cp cp, afqmc_walkers w, int which);
  * walker: diag4 = nw x 4 x L complex (diag00, diag01, diag10, diag11), sample_logw nw complex.
  * Dense (spin_blocks = 0) sets only. */
 int afqmc_cp_hs_apply(afqmc_cp cp, afqmc_walkers w, const double* diag4, const double* sample_logw);
-/* SD::stabilize (SD.cpp:42-45) for every alive walker (CholeskyQR2, R with positive diagonal) */
+/* SD::stabilize (SD.cpp:42-45) for every alive walker: CholeskyQR in FP64, R with positive diagonal,
+ * a second pass for walkers not yet orthonormal; a walker whose Gram matrix is not numerically positive
+ * definite (condition number beyond about 1e8) is redone by shifted CholeskyQR3, and one whose shifted
+ * Gram matrix still breaks down (rank deficient: Householder's R has a zero on its diagonal, log det R =
+ * -inf, weight zero) is marked dead */
 int afqmc_cp_stabilize(afqmc_cp cp, afqmc_walkers w);
+/* walkers that took the shifted CholeskyQR and walkers marked dead as rank deficient, summed over all
+ * stabilisations of this propagator */
+int afqmc_cp_get_qr_stats(afqmc_cp cp, uint64_t* shifted, uint64_t* killed);
 /* popConfiguration(1, weight) (populationControlConfiguration.cpp:24-101), one uniform per slot;
  * stats = popCheck min, max, mean (may be NULL) */
 int afqmc_comb(afqmc_ctx ctx, int nw, const double* weight, const double* uniforms, int* table, double* stats);
@@ -157,6 +172,16 @@ int afqmc_cp_apply_table(afqmc_cp cp, afqmc_walkers w, const int* table);
 int afqmc_cp_step(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out);
 /* popControl (:103-156): weights (nw), table (nw), stats (3) may be NULL */
 int afqmc_cp_pop_control(afqmc_cp cp, afqmc_walkers w, double* weights_out, int* table_out, double* stats);
+/* projectExpMinusDtKExpMinusDtV (:37-74) with each walker's aux fields given (nw x L ints, +-1; rows of
+ * dead walkers are ignored) instead of sampled: no uniforms are drawn, while the force bias still enters
+ * the sample's logw (getTwoBodySampleFromAuxForce, NiupNidn.cpp:133-140) and the phase projection still
+ * applies.  Replays the fields of another run (aux_out of afqmc_cp_step), e.g. the FP64 path's fields
+ * in the mixed path on the same walkers, which isolates the precision error of the energy. */
+int afqmc_cp_step_fields(afqmc_cp cp, afqmc_walkers w, const int* aux);
+/* popControl (:103-156) with the comb's table given (nw ints, e.g. the table_out of another run; every
+ * parent must keep its own slot) instead of drawn: the weights are formed and the log-weights shifted
+ * by -logO as usual (weights_out nw, may be NULL); no uniforms are drawn. */
+int afqmc_cp_pop_control_table(afqmc_cp cp, afqmc_walkers w, const int* table, double* weights_out);
 /* addMixedMeasurement (afqmcConstraintPathMeasure.cpp:9-27) */
 int afqmc_cp_measure(afqmc_cp cp, afqmc_walkers w);
 /* accumulated den, HNum, KNum, VNum, RNum, NupTotNum, NdnTotNum: 7 complex */
@@ -292,8 +317,10 @@ int afqmc_ph_sample_matrix(afqmc_ph ph, afqmc_ph_walkers w, double* M);
 int afqmc_ph_taylor_orders(afqmc_ph ph, afqmc_ph_walkers w, int* orders);
 /* Hop2isSD2isOperation / LogHop2isSD2isOperation applyToRight with one-body operator `which` */
 int afqmc_ph_onebody(afqmc_ph ph, afqmc_ph_walkers w, int which);
-/* SD2is::stabilize (SD2is.cpp:48-61) by CholeskyQR2 */
+/* SD2is::stabilize (SD2is.cpp:48-61) by CholeskyQR in FP64, as afqmc_cp_stabilize (second pass where
+ * needed, shifted CholeskyQR3 for ill-conditioned walkers, rank-deficient walkers marked dead) */
 int afqmc_ph_stabilize(afqmc_ph ph, afqmc_ph_walkers w);
+int afqmc_ph_get_qr_stats(afqmc_ph ph, uint64_t* shifted, uint64_t* killed);
 /* projectExpMinusDtKExpMinusDtV (afqmcPhaselessProjection.cpp:75-121): force and aux records nw x G complex
  * (may be NULL) */
 int afqmc_ph_step(afqmc_ph ph, afqmc_ph_walkers w, double* force_out, double* aux_out);

@@ -295,3 +295,93 @@ def test_driver_trajectory_matches_oracle(mode):
     for a, b in zip(dev.energies, ref.energies):
         for x, y in zip(a[:2], b[:2]):
             assert abs(x - y) < 1e-9 * abs(y), (x, y)
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_step_fields_and_table_replay_reproduce_the_run(mode, precision):
+    """afqmc_cp_step_fields with the fields a step recorded, and afqmc_cp_pop_control_table with
+    the table a comb drew, reproduce that step and comb bit for bit on identical walkers, and
+    draw no uniforms."""
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode, nw=12, precision=precision)
+    prop2 = CPPropagator(ctx(), model, method, phi, precision=precision, max_walkers=12,
+                         **(dict(spin_blocks=False) if mode == "dense" else dict(nup=5, spin_blocks=True)))
+    ws2 = prop2.walkers(12)
+    ws2.upload(wfs, logw)
+    for p in (prop, prop2):
+        p.set_seed(3, 0)
+        p.set_ET(-7.0)
+    for _ in range(3):
+        f, aux = prop.step(ws, record=True)
+        c0 = prop2.counter()
+        al = ws2.alive()
+        aux = np.where(al[:, None] != 0, aux, 0)  # rows of dead walkers are not read
+        prop2.step_fields(ws2, aux)
+        assert prop2.counter() == c0
+        a, b = ws.download(), ws2.download()
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    wt, tb, st = prop.pop_control(ws, record=True)
+    wt2 = prop2.pop_control_table(ws2, tb)
+    assert np.array_equal(wt, wt2)
+    a, b = ws.download(), ws2.download()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    with pytest.raises(Exception):
+        prop2.step_fields(ws2, np.zeros((12, model.L), dtype=np.int32))
+    bad = np.arange(12, dtype=np.int32)
+    bad[0], bad[1] = 1, 2  # slot 1 is a parent but does not keep itself
+    with pytest.raises(Exception):
+        prop2.pop_control_table(ws2, bad)
+
+
+@pytest.mark.parametrize("mode", ["dense", "blocks"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_stabilize_ill_conditioned_and_rank_deficient(mode, precision):
+    """Walkers beyond CholeskyQR: FP64 walkers with two columns 1e-10 apart (condition number
+    about 1e10, the Gram matrix is not positive definite in FP64) take the shifted CholeskyQR3
+    path, with Q orthonormal and log det R equal to the oracle's Householder QR; a walker with two
+    equal columns (numerically rank deficient) stays alive with an orthonormal Q and a log-weight
+    lowered to the noise level of its R_kk, as Householder lowers it; a walker with a zero column
+    (Householder's R_kk = 0, log det R = -inf) is marked dead and counted; the well-conditioned
+    walkers beside them match the oracle as before."""
+    model, om, phi, prop, ws, wfs, logw, method = make_case(mode, precision=precision)
+    rows = slice(0, 2 * model.L) if mode == "dense" else slice(0, model.L)
+    rng = np.random.default_rng(17)
+    dup, zero = 5, 4
+    ill = {1, 7, 10} if precision == 0 else set()
+    wfs2 = []
+    for i, w in enumerate(wfs):
+        w = w.copy()
+        n = w[rows, 0].shape[0]
+        if i in ill:
+            w[rows, 1] = w[rows, 0] + 1e-10 * crand(rng, n)
+        if i == dup:
+            w[rows, 2] = w[rows, 0]
+        if i == zero:
+            w[rows, 2] = 0.0
+        wfs2.append(w)
+    ws.upload(wfs2, logw)
+    prop.stabilize(ws)
+    out, lw, al = ws.download()
+    shifted, killed = prop.qr_stats()
+    assert killed == 1 and al[zero] == 0
+    assert shifted == len(ill) + 2
+    for i in range(ws.nw):
+        if i == zero:
+            continue
+        assert al[i] == 1, i
+        q = out[i]
+        if precision == 0:
+            assert np.max(np.abs(q.conj().T @ q - np.eye(model.N))) < 1e-12, i
+        if i == dup:
+            good = SD(wf=wfs[i], logw=logw[i])
+            good.stabilize()
+            assert lw[i].real < good.logw.real - 20.0
+            continue
+        ref = SD(wf=wfs2[i], logw=logw[i])
+        ref.stabilize()
+        tol = (1e-5 if i in ill else RTOL) if precision == 0 else 1e-5
+        assert abs(lw[i].real - ref.logw.real) < tol * max(1, abs(ref.logw.real)), i
+        D = q.conj().T @ ref.wf
+        assert np.max(np.abs(np.abs(np.diag(D)) - 1)) < (1e-5 if (i in ill or precision) else 1e-10), i
