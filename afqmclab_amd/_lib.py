@@ -85,6 +85,7 @@ SIGNATURES = {
     "afqmc_walkers_download_some": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip]),
     "afqmc_walkers_set_alive": (C.c_int, [_vp, _ip]),
     "afqmc_cp_walkers_from_trial": (C.c_int, [_vp, _vp, _dp]),
+    "afqmc_comb_noalloc": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
     "afqmc_cp_apply_table": (C.c_int, [_vp, _vp, _ip]),
     "afqmc_cp_step": (C.c_int, [_vp, _vp, _dp, _ip]),
     "afqmc_cp_pop_control": (C.c_int, [_vp, _vp, _dp, _ip, _dp]),

@@ -132,11 +132,18 @@ int afqmc_ctx_create(int device, afqmc_ctx* out) {
   hipGetDevice(&prev);
   hipSetDevice(device);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  // the comb scratch array's address on this device, looked up once with the device current
+  const int src = e == hipSuccess ? afqmc_noalloc_scratch_init(device) : 0;
   hipSetDevice(prev);
   if (e != hipSuccess) {
     delete c;
     afqmc_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
     return -2;
+  }
+  if (src) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return src;
   }
   *out = c;
   return 0;

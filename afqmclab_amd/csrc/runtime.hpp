@@ -144,6 +144,9 @@ void swap(DevBuf<T>& a, DevBuf<T>& b) noexcept {
   std::swap(a.n, b.n);
 }
 
+// looks up the device's copy of the comb scratch array used by afqmc_comb_noalloc (device current)
+int afqmc_noalloc_scratch_init(int device);
+
 int launch_convert_f32_f64(const void* in, void* out, long long ncomplex, hipStream_t s);
 int launch_convert_f64_f32(const void* in, void* out, long long ncomplex, hipStream_t s);
 // logw_out[w] = logw[w] + c

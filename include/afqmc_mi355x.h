@@ -163,6 +163,16 @@ int afqmc_cp_get_qr_stats(afqmc_cp cp, uint64_t* shifted, uint64_t* killed);
 /* popConfiguration(1, weight) (populationControlConfiguration.cpp:24-101), one uniform per slot;
  * stats = popCheck min, max, mean (may be NULL) */
 int afqmc_comb(afqmc_ctx ctx, int nw, const double* weight, const double* uniforms, int* table, double* stats);
+/* afqmc_comb on device buffers, allocating nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  weight and uniforms (nw doubles each), table (nw ints) and
+ * stats (3 doubles, may be NULL) are device pointers.  The comb's working memory (run scalars,
+ * probabilities, slot counts) is a fixed device array of about 0.8 MB per GPU, held until the
+ * process ends, so nw is limited to AFQMC_COMB_NOALLOC_MAX.  The call is enqueued on the
+ * context's stream and returns without synchronising.  All calls on one GPU share that array:
+ * calls that overlap in time on one GPU (other streams, other contexts, other threads) race. */
+#define AFQMC_COMB_NOALLOC_MAX 65536
+int afqmc_comb_noalloc(afqmc_ctx ctx, int nw, const double* weight, const double* uniforms, int* table, double* stats);
 /* populationControl, serial branch (populationControl.h:110-116): walker[i] = walker[table[i]] */
 int afqmc_cp_apply_table(afqmc_cp cp, afqmc_walkers w, const int* table);
 
