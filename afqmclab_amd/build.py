@@ -17,6 +17,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libafqmc_mi355x.so")
 HOST = os.path.join(HERE, "host")
 CP_RUN = os.path.join(HERE, "afqmc_cp_run")
+PH_RUN = os.path.join(HERE, "afqmc_ph_run")
+# the HDF5 C library for the phaseless driver's model_param (the reference's RealMaterialMolecule file)
+HDF5_PREFIX = os.environ.get("AFQMC_HDF5_PREFIX", "/opt/conda")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AFQMC_OFFLOAD_ARCH", "gfx950")
@@ -69,8 +72,26 @@ def build(verbose=True, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"host driver build failed:\n{r.stderr}")
     os.replace(CP_RUN + ".tmp", CP_RUN)
+    built = [CP_RUN]
+    # phaseless driver: opens the HDF5 C library at run time for model_param; skipped (with a note)
+    # when the headers and library are absent
+    h5inc, h5lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if os.path.exists(os.path.join(h5inc, "hdf5.h")) and os.path.exists(os.path.join(h5lib, "libhdf5.so")):
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+               "-I", os.path.join(HERE, "..", "include"), "-I", os.path.join(rocm, "include"), "-isystem", h5inc,
+               f'-DAFQMC_HDF5_LIB="{os.path.realpath(os.path.join(h5lib, "libhdf5.so"))}"',
+               os.path.join(HOST, "ph_run.cpp"), "-o", PH_RUN + ".tmp", "-L", HERE, "-lafqmc_mi355x",
+               "-L", os.path.join(rocm, "lib"), "-lhipsolver", "-lamdhip64", "-ldl",
+               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"phaseless host driver build failed:\n{r.stderr}")
+        os.replace(PH_RUN + ".tmp", PH_RUN)
+        built.append(PH_RUN)
+    elif verbose:
+        print(f"HDF5 not found under {HDF5_PREFIX}: afqmc_ph_run (HDF5 model_param) not built")
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources and {CP_RUN}")
+        print(f"built {LIB} from {len(srcs)} sources and {', '.join(built)}")
     return LIB
 
 

@@ -21,7 +21,8 @@
 using cd = std::complex<double>;
 
 namespace {
-constexpr int TAYLOR_MAX_ORDER = 60;
+// the series terms fall as |A|^k / k!, so a term count this large means a runaway operator
+constexpr int TAYLOR_MAX_ORDER = 200;
 
 template <typename T>
 __global__ void transpose_half_rot(int L, int Ns, int G, const cplx<double>* in, cplx<T>* out) {
@@ -59,7 +60,7 @@ struct afqmc_ph_s {
   // workspace
   DevBuf<char> ovl[2], X[2], bgbuf, xs, Msamp, Tk[2], Dbuf, qrS, qr64, qr64b;
   DevBuf<zd> ldet[2], logO, den, tkin, ex, per_walker, bg_acc, ex_acc, frec, auxrec, slw, qrld;
-  DevBuf<int> info[2], prefix, table, comb_num, active, orders, nactive, qrinfo, qr_need, qr_flag;
+  DevBuf<int> info[2], prefix, table, comb_num, active, orders, nactive, qrinfo, qr_need, qr_flag, lead;
   DevBuf<double> weight, comb_prob;
   int dchunk = 1;
   size_t esz() const { return fp32 ? sizeof(zf) : sizeof(zd); }
@@ -158,11 +159,12 @@ int ph_sample_matrix(afqmc_ph ph, afqmc_ph_walkers w) {
   return launch_cgemm<T>(OP_NR, EPI_STORE, a, ph->ctx->stream);
 }
 
-// W <- sum_k A^k W / k! with per-walker dynamic order (LogHop2isSD2isOperation::applyToRight);
-// A per walker (strideA = L^2) or shared (strideA = 0).  Result written back to wf.
+// tmp <- sum_k A^k W / k! for the walkers in mask: `base` terms, then (dyn) more until the largest
+// element of the last term is at most the accuracy (LogHop2isSD2isOperation.cpp:99-195); A per walker
+// (strideA = L^2) or shared (strideA = 0).  maxo: the largest number of terms taken.
 template <typename T>
-int ph_taylor(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, const void* A, long long strideA, const int* alive,
-              int* max_order_out) {
+int ph_taylor_pass(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, const void* A, long long strideA,
+                   const int* mask, bool dyn, int base, int* maxo) {
   hipStream_t s = ph->ctx->stream;
   const long long n = w->wsize();
   const size_t bytes = ph->esz() * n * w->nw;
@@ -170,11 +172,12 @@ int ph_taylor(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, const void* A, 
   // tmp <- W (the accumulated sum), Tk[0] <- W (the current term)
   HIP_TRY(hipMemcpyAsync(w->tmp.p, wf.p, bytes, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(ph->Tk[0].p, wf.p, bytes, hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipMemcpyAsync(ph->active.p, alive, sizeof(int) * w->nw, hipMemcpyDeviceToDevice, s));
-  const bool dyn = ph->tay_dynamic || ph->tay_order == 0;
-  const int base = dyn ? ph->tay_base : ph->tay_order;
-  int cur = 0, maxo = 0;
-  for (int m = 1; m <= TAYLOR_MAX_ORDER; ++m) {
+  HIP_TRY(hipMemcpyAsync(ph->active.p, mask, sizeof(int) * w->nw, hipMemcpyDeviceToDevice, s));
+  int cur = 0;
+  *maxo = 0;
+  for (int m = 1;; ++m) {
+    REQUIRE(m <= TAYLOR_MAX_ORDER, "the Taylor series of exp(A) W has not converged in %d terms (accuracy %g)",
+            TAYLOR_MAX_ORDER, ph->tay_acc);
     GemmArgs a = gbase();
     a.M = ph->L; a.N = ph->Nmax; a.K = ph->L; a.batch = w->nw;
     a.A = A; a.lda = ph->L; a.strideA = strideA;
@@ -188,7 +191,7 @@ int ph_taylor(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, const void* A, 
                                           ph->tay_acc, dyn ? 1 : 0, ph->nactive.p, s)))
       return rc;
     cur ^= 1;
-    maxo = m;
+    *maxo = m;
     if (m >= base) {
       int h = 0;
       HIP_TRY(hipMemcpyAsync(&h, ph->nactive.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -196,10 +199,43 @@ int ph_taylor(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, const void* A, 
       if (h == 0) break;
     }
   }
+  return 0;
+}
+
+// W <- sum_k A^k W / k! (LogHop2isSD2isOperation::applyToRight) for the alive walkers; result written
+// back to wf.  dynamicOrder: each walker's own order.  fixedOrder with a given order: that many
+// terms.  fixedOrder with order 0: as the reference's serial run (determinantAndAddFixedOrders,
+// logHop2isSD2isOperation.cpp:122-133), the first application finds the dynamic order of its first
+// walker (here the first alive walker), freezes it, and every walker takes that many terms.
+template <typename T>
+int ph_taylor(afqmc_ph ph, afqmc_ph_walkers w, DevBuf<char>& wf, const void* A, long long strideA, const int* alive,
+              int* max_order_out) {
+  hipStream_t s = ph->ctx->stream;
+  const long long n = w->wsize();
+  int rc, maxo = 0;
+  if (!ph->tay_dynamic && ph->tay_order == 0) {
+    std::vector<int> al(w->nw), lead(w->nw, 0);
+    HIP_TRY(hipMemcpyAsync(al.data(), alive, sizeof(int) * w->nw, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int first = -1;
+    for (int i = 0; i < w->nw && first < 0; ++i)
+      if (al[i]) first = i;
+    if (first < 0) {  // no walker to propagate: the order stays to be found
+      if (max_order_out) *max_order_out = 0;
+      return 0;
+    }
+    lead[first] = 1;
+    HIP_TRY(hipMemcpyAsync(ph->lead.p, lead.data(), sizeof(int) * w->nw, hipMemcpyHostToDevice, s));
+    if ((rc = ph_taylor_pass<T>(ph, w, wf, A, strideA, ph->lead.p, true, ph->tay_base, &maxo))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));  // the host copy of the mask must outlive the copy
+    ph->tay_order = maxo;
+  }
+  const bool dyn = ph->tay_dynamic != 0;
+  const int base = dyn ? ph->tay_base : ph->tay_order;
+  if ((rc = ph_taylor_pass<T>(ph, w, wf, A, strideA, alive, dyn, base, &maxo))) return rc;
   // alive walkers take the sum; killed ones keep their state (the reference skips them)
   if ((rc = launch_select_alive(w->nw, n, ph->esz(), alive, w->tmp.p, wf.p, s))) return rc;
   if (max_order_out) *max_order_out = maxo;
-  if (ph->tay_order == 0 && !ph->tay_dynamic) ph->tay_order = maxo;  // fixedOrder determined on first use
   return 0;
 }
 
@@ -499,6 +535,7 @@ int afqmc_ph_create(afqmc_ctx ctx, const afqmc_ph_desc* d, afqmc_ph* out) {
   if (!rc) rc = ph->qrinfo.alloc(nw);
   if (!rc) rc = ph->qr_need.alloc(nw);
   if (!rc) rc = ph->qr_flag.alloc(nw);
+  if (!rc) rc = ph->lead.alloc(nw);
   if (!rc) rc = ph->qr64.alloc_bytes(sizeof(zd) * wsz * nw);
   if (!rc) rc = ph->qr64b.alloc_bytes(sizeof(zd) * wsz * nw);
   if (!rc) rc = ph->logO.alloc(nw);

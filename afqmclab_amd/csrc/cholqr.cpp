@@ -63,8 +63,9 @@ int refine(const CholQrBlock& q, const int* among, hipStream_t s) {
   if ((rc = gram(q, among, s))) return rc;
   if ((rc = launch_ortho_check(q.nw, q.N, q.S, among, q.tol, q.need, s))) return rc;
   if ((rc = chol(q, q.need, 0.0, s))) return rc;
-  // an exactly zero column left after the shifted pass (Householder's R_kk = 0): the walker dies
-  if ((rc = launch_qr_kill(q.nw, q.need, q.info, q.alive, q.sc, 0, s))) return rc;
+  // an exactly zero column left after the shifted pass (Householder's R_kk = 0): the walker dies, and
+  // leaves the shifted set so that the pass over it does not count the walker a second time
+  if ((rc = launch_qr_kill(q.nw, q.need, q.flag, q.info, q.alive, q.sc, 0, s))) return rc;
   if ((rc = launch_select_alive(q.nw, (long long)q.M * q.N, sizeof(zd), q.need, q.wf->p, q.tmp->p, s))) return rc;
   if ((rc = times_rinv(q, q.tmp->p, q.wf->p, q.need, s))) return rc;
   return launch_logw_add_real(q.nw, q.logw, q.need, q.ld, nullptr, 0, q.m0, q.m1, s);
@@ -85,7 +86,7 @@ int cholqr_f64(const CholQrBlock& q, hipStream_t s) {
   if ((rc = launch_qr_shift(q.nw, q.N, q.S, q.flag, c, s))) return rc;
   if ((rc = chol(q, q.flag, 0.0, s))) return rc;
   // the shifted Gram matrix is positive definite unless the walker holds non-finite numbers
-  if ((rc = launch_qr_kill(q.nw, q.flag, q.info, q.alive, q.sc, 1, s))) return rc;
+  if ((rc = launch_qr_kill(q.nw, q.flag, nullptr, q.info, q.alive, q.sc, 1, s))) return rc;
   if ((rc = times_rinv(q, q.wf->p, q.tmp->p, q.alive, s))) return rc;
   std::swap(*q.wf, *q.tmp);
   if ((rc = launch_logw_add_real(q.nw, q.logw, q.alive, q.ld, nullptr, 0, q.m0, q.m1, s))) return rc;

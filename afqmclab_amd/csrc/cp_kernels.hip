@@ -3,6 +3,9 @@
 // HubbardSOCMeasureFixedSDSD (:178-254) and the comb (populationControlConfiguration.cpp:24-101).
 #include "cp_kernels.hpp"
 
+#include <algorithm>
+#include <type_traits>
+
 unsigned long long afqmc_seed_mix_host(unsigned long long seed) {
   unsigned long long z = seed * 0x9E3779B97F4A7C15ull + 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -269,20 +272,48 @@ __global__ void advance_counter_kernel(DevScalars* sc, int per_walker) {
 }
 
 // ------------------------------------------------------------------ HS 2x2 apply
-template <typename T>
-__global__ void hs_apply_2x2_kernel(int nw, int L, int N, const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,
-                                    const cplx<T>* __restrict__ d) {
-  const long long per = (long long)L * N;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= per * nw) return;
-  const int w = (int)(idx / per);
-  const long long r = idx - (long long)w * per;
-  const int j = (int)(r / L), i = (int)(r - (long long)j * L);
-  const long long o = (long long)w * 2 * per + (long long)j * 2 * L + i;
-  const cplx<T>* dw = d + (long long)w * 4 * L;
-  const cplx<T> a = in[o], b = in[o + L];
-  out[o] = cadd(cmul(dw[i], a), cmul(dw[L + i], b));
-  out[o + L] = cadd(cmul(dw[2 * L + i], a), cmul(dw[3 * L + i], b));
+// NiupNidnSampleSDOperation::applyToRight (NiupNidnSDOperation.cpp:15-41) on dense 2L x N walkers:
+// for every column j and site i, (w'(i), w'(i+L)) = [d00 d01; d10 d11]_i (w(i), w(i+L)).  Grid
+// (column chunk, walker): the walker's 4 x L site matrices are staged in LDS once per workgroup,
+// and each work item moves S consecutive sites of one column as 16-byte vectors (S = 2 complex
+// FP32 or 1 complex FP64; S = 1 with 8-byte FP32 accesses when L is odd).  No 64-bit division.
+template <typename T, int S>
+__global__ __launch_bounds__(256) void hs_apply_2x2_kernel(int L, int N, int cpb, const cplx<T>* __restrict__ in,
+                                                           cplx<T>* __restrict__ out, const cplx<T>* __restrict__ d) {
+  typedef cplx<T> ct;
+  struct alignas(S * sizeof(ct)) vt { ct z[S]; };
+  extern __shared__ __align__(16) unsigned char hs_lds[];
+  vt* sd = reinterpret_cast<vt*>(hs_lds);
+  const int w = blockIdx.y, tid = threadIdx.x;
+  const int nv = L / S;  // site vectors per half column
+  const vt* dw = reinterpret_cast<const vt*>(d + (long long)w * 4 * L);
+  for (int k = tid; k < 4 * nv; k += blockDim.x) sd[k] = dw[k];
+  __syncthreads();
+  const int j0 = blockIdx.x * cpb, j1 = min(N, j0 + cpb);
+  const long long base = (long long)w * 2 * L * N;
+  auto item = [&](int j, int v) {
+    const long long o = base + (long long)j * 2 * L + (long long)v * S;
+    const vt a = *reinterpret_cast<const vt*>(in + o);
+    const vt b = *reinterpret_cast<const vt*>(in + o + L);
+    const vt d00 = sd[v], d01 = sd[nv + v], d10 = sd[2 * nv + v], d11 = sd[3 * nv + v];
+    vt up, dn;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      up.z[t] = cadd(cmul(d00.z[t], a.z[t]), cmul(d01.z[t], b.z[t]));
+      dn.z[t] = cadd(cmul(d10.z[t], a.z[t]), cmul(d11.z[t], b.z[t]));
+    }
+    *reinterpret_cast<vt*>(out + o) = up;
+    *reinterpret_cast<vt*>(out + o + L) = dn;
+  };
+  if (nv <= (int)blockDim.x) {
+    // several columns per pass: thread -> (column offset, site vector), fixed over the loop
+    const int cpp = blockDim.x / nv, jj = tid / nv, v = tid - jj * nv;
+    if (jj < cpp)
+      for (int j = j0 + jj; j < j1; j += cpp) item(j, v);
+  } else {
+    for (int j = j0; j < j1; ++j)
+      for (int v = tid; v < nv; v += blockDim.x) item(j, v);
+  }
 }
 
 // ------------------------------------------------------------------ comb
@@ -576,7 +607,8 @@ __global__ void qr_shift_kernel(int N, zd* S, const int* flag, double c) {
 
 // one workgroup: flagged walkers count as shifted, those whose shifted Cholesky broke down as well
 // die (their Gram matrix is singular: a rank-deficient walker); the counts are added by one thread
-__global__ void qr_kill_kernel(int nw, int* flag, const int* info, int* alive, DevScalars* sc, int count_shifted) {
+__global__ void qr_kill_kernel(int nw, int* flag, int* flag2, const int* info, int* alive, DevScalars* sc,
+                               int count_shifted) {
   __shared__ double sh[16];
   double nshift = 0.0, nkill = 0.0;
   for (int w = threadIdx.x; w < nw; w += blockDim.x) {
@@ -585,6 +617,7 @@ __global__ void qr_kill_kernel(int nw, int* flag, const int* info, int* alive, D
     if (info[w]) {
       alive[w] = 0;
       flag[w] = 0;
+      if (flag2) flag2[w] = 0;
       nkill += 1.0;
     }
   }
@@ -837,11 +870,24 @@ int launch_advance_counter(DevScalars* sc, int per_walker, hipStream_t s) {
 
 template <typename T>
 int launch_hs_apply_2x2(int nw, int L, int N, const void* in, void* out, const void* d2x2, hipStream_t s) {
-  long long tot = (long long)nw * L * N;
-  if (tot == 0) return 0;
-  hipLaunchKernelGGL(hs_apply_2x2_kernel<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nw, L, N,
-                     reinterpret_cast<const cplx<T>*>(in), reinterpret_cast<cplx<T>*>(out),
-                     reinterpret_cast<const cplx<T>*>(d2x2));
+  if ((long long)nw * L * N == 0) return 0;
+  // 16-byte vectors of two FP32 complex numbers need an even L (every row offset then even)
+  const bool pair = std::is_same<T, float>::value && L % 2 == 0;
+  const int S = pair ? 2 : 1;
+  const size_t lds = (size_t)4 * L * sizeof(cplx<T>);
+  // about 64 site vectors per thread per workgroup
+  const int nv = L / S;
+  const int cpb = std::max(1, std::min(N, (64 * 256) / std::max(nv, 1)));
+  dim3 grid((unsigned)((N + cpb - 1) / cpb), (unsigned)nw);
+  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in);
+  const cplx<T>* dd = reinterpret_cast<const cplx<T>*>(d2x2);
+  cplx<T>* dst = reinterpret_cast<cplx<T>*>(out);
+  if constexpr (std::is_same<T, float>::value) {
+    if (pair) hipLaunchKernelGGL((hs_apply_2x2_kernel<T, 2>), grid, dim3(256), lds, s, L, N, cpb, src, dst, dd);
+    else hipLaunchKernelGGL((hs_apply_2x2_kernel<T, 1>), grid, dim3(256), lds, s, L, N, cpb, src, dst, dd);
+  } else {
+    hipLaunchKernelGGL((hs_apply_2x2_kernel<T, 1>), grid, dim3(256), lds, s, L, N, cpb, src, dst, dd);
+  }
   HIP_LAUNCH_CHECK();
   return 0;
 }
@@ -904,10 +950,11 @@ int launch_qr_shift(int nw, int N, zd* S, const int* flag, double c, hipStream_t
   return 0;
 }
 
-int launch_qr_kill(int nw, int* flag, const int* info, int* alive, DevScalars* sc, int count_shifted,
-                   hipStream_t s) {
+int launch_qr_kill(int nw, int* flag, int* flag2, const int* info, int* alive, DevScalars* sc,
+                   int count_shifted, hipStream_t s) {
   if (nw <= 0) return 0;
-  hipLaunchKernelGGL(qr_kill_kernel, dim3(1), dim3(1024), 0, s, nw, flag, info, alive, sc, count_shifted);
+  hipLaunchKernelGGL(qr_kill_kernel, dim3(1), dim3(1024), 0, s, nw, flag, flag2, info, alive, sc,
+                     count_shifted);
   HIP_LAUNCH_CHECK();
   return 0;
 }

@@ -114,10 +114,11 @@ int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol,
 int launch_qr_flag(int nw, const int* alive, const int* info, int* flag, hipStream_t s);
 // S_w += c * tr(S_w) * I for the flagged walkers (the shift of shifted CholeskyQR)
 int launch_qr_shift(int nw, int N, zd* S, const int* flag, double c, hipStream_t s);
-// flagged walkers whose Cholesky broke down (info) die and leave the flag; counts into sc (the flagged
-// ones as shifted when count_shifted)
-int launch_qr_kill(int nw, int* flag, const int* info, int* alive, DevScalars* sc, int count_shifted,
-                   hipStream_t s);
+// flagged walkers whose Cholesky broke down (info) die and leave the flag (and flag2 when given, so a
+// later pass over flag2 does not see them again); counts into sc (the flagged ones as shifted when
+// count_shifted)
+int launch_qr_kill(int nw, int* flag, int* flag2, const int* info, int* alive, DevScalars* sc,
+                   int count_shifted, hipStream_t s);
 int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int use_imag_sub,
                          double mult0, double mult1, hipStream_t s);
 

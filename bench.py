@@ -57,7 +57,7 @@ def flops_per_walker_step(Ls, nup, ndn, mgs, pop, meas):
     step = M2N + 2 * MN2 + N3 / 3.0 + 3 * MN          # propagate + overlap + green + LU
     qr = (2 * MN2 - 2.0 / 3.0 * N3) / mgs
     comb = (MN2 + N3 / 3.0) / pop
-    measure = (M2N + 2 * MN2 + N3 / 3.0 + 4 * MN) / meas
+    measure = (M2N + 2 * MN2 + N3 / 3.0 + 4 * MN) / meas if meas else 0.0
     return 8.0 * (step + qr + comb + measure), 8.0 * sum(Ls * Ls * n for n in (nup, ndn))
 
 
@@ -291,6 +291,13 @@ def hetero_instances(n_each=6, seed=20241):
     N rounded to the nearest even number (Nup = Ndn), U uniform in [2, 8], t' = -0.2 on odd ones."""
     from afqmclab_amd.model import heterogeneous_hubbard_instances
     return heterogeneous_hubbard_instances(n_each, seed)
+
+
+def hetero_roofline(instances, nw, steps_per_s):
+    flops = sum(flops_per_walker_step(model.L, nup, nup, 10, 40, None)[0] * nw for _, model, _, nup in instances)
+    achieved = flops * steps_per_s / 1e12
+    return {"bound": "mfma", "flops_per_step": flops, "achieved": achieved, "peak": PEAK_TFLOPS["mixed"],
+            "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["mixed"]}
 
 
 def run_hetero(instances, nw, steps, seed, tuned_graph):
@@ -708,6 +715,9 @@ def main():
             "graph_autotuned": {"walker_steps_per_s": graph["walker_steps_per_s"], "seconds": graph["elapsed"],
                                 "graph_nodes_per_block": graph["graph_nodes"], "tiles": graph["tiles"]},
             "speedup": graph["walker_steps_per_s"] / eager["walker_steps_per_s"],
+            # algorithmic flops of all instances per step (spin-block form, QR every 10, comb every
+            # 40, no measurement) over the graph path's time per step, against the FP32 MFMA peak
+            "roofline": hetero_roofline(inst, 256, graph["walker_steps_per_s"] / (len(inst) * 256)),
             "same_log_overlaps": bool(same),
         }
     if args.hs_steps > 0:

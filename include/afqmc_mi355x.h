@@ -283,7 +283,9 @@ typedef struct {
   const double* const_force;  /* G complex, used when force_type == 1 (NULL = zeros) */
   int max_walkers;
   int taylor_fixed;           /* 0 dynamicOrder, 1 fixedOrder */
-  int taylor_order;           /* fixedOrder: terms (0 = found on the first application, then frozen) */
+  int taylor_order;           /* fixedOrder: terms (0 = the first application takes the dynamic order of its
+                                 first alive walker and freezes it for every walker, as the serial
+                                 reference's determinantAndAddFixedOrders, logHop2isSD2isOperation.cpp:122-133) */
   int taylor_base;            /* baseTaylorOrder (<= 0: 3) */
   double taylor_accuracy;     /* <= 0: 1e-10 */
 } afqmc_ph_desc;

@@ -340,11 +340,12 @@ def test_step_fields_and_table_replay_reproduce_the_run(mode, precision):
 def test_stabilize_ill_conditioned_and_rank_deficient(mode, precision):
     """Walkers beyond CholeskyQR: FP64 walkers with two columns 1e-10 apart (condition number
     about 1e10, the Gram matrix is not positive definite in FP64) take the shifted CholeskyQR3
-    path, with Q orthonormal and log det R equal to the oracle's Householder QR; a walker with two
-    equal columns (numerically rank deficient) stays alive with an orthonormal Q and a log-weight
-    lowered to the noise level of its R_kk, as Householder lowers it; a walker with a zero column
-    (Householder's R_kk = 0, log det R = -inf) is marked dead and counted; the well-conditioned
-    walkers beside them match the oracle as before."""
+    path, with Q orthonormal and log det R equal to the oracle's Householder QR; a walker with a
+    zero column (Householder's R_kk = 0, log det R = -inf) is marked dead and counted; a walker
+    with two equal columns is numerically rank deficient: where the shifted pass leaves its
+    column at the rounding level it dies as well, otherwise it stays alive with an orthonormal Q
+    and a log-weight lowered to the noise level of its R_kk, as Householder lowers it; the
+    well-conditioned walkers beside them match the oracle as before."""
     model, om, phi, prop, ws, wfs, logw, method = make_case(mode, precision=precision)
     rows = slice(0, 2 * model.L) if mode == "dense" else slice(0, model.L)
     rng = np.random.default_rng(17)
@@ -365,10 +366,11 @@ def test_stabilize_ill_conditioned_and_rank_deficient(mode, precision):
     prop.stabilize(ws)
     out, lw, al = ws.download()
     shifted, killed = prop.qr_stats()
-    assert killed == 1 and al[zero] == 0
+    assert al[zero] == 0
+    assert killed == 1 + (al[dup] == 0)
     assert shifted == len(ill) + 2
     for i in range(ws.nw):
-        if i == zero:
+        if i == zero or (i == dup and al[dup] == 0):
             continue
         assert al[i] == 1, i
         q = out[i]
@@ -385,3 +387,35 @@ def test_stabilize_ill_conditioned_and_rank_deficient(mode, precision):
         assert abs(lw[i].real - ref.logw.real) < tol * max(1, abs(ref.logw.real)), i
         D = q.conj().T @ ref.wf
         assert np.max(np.abs(np.abs(np.diag(D)) - 1)) < (1e-5 if (i in ill or precision) else 1e-10), i
+
+
+@pytest.mark.parametrize("lattice", [(3, 5), (24, 24), (16, 16)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_hs_apply_2x2_shapes(lattice, precision):
+    """The spin-mixing HS kernel on an odd site count (8-byte FP32 accesses), on more sites than a
+    workgroup has threads (several passes per column) and on the 16 x 16 lattice (several columns
+    per pass), against NiupNidnSampleSDOperation::applyToRight of the oracle."""
+    nx, ny = lattice
+    L = nx * ny
+    N = min(2 * L - 2, 24)
+    model = HubbardModel.square(nx, ny, N, 4.0)
+    phi = free_electron_trial(model)
+    method = DevMethod(dt=0.05, walkerSizePerThread=5)
+    prop = CPPropagator(ctx(), model, method, phi, spin_blocks=False, precision=precision, max_walkers=5)
+    rng = np.random.default_rng(L)
+    wfs = [phi + 0.3 * crand(rng, 2 * L, N) for _ in range(5)]
+    logw = 0.1 * crand(rng, 5)
+    ws = prop.walkers(5)
+    ws.upload(wfs, logw)
+    d4 = crand(rng, 5, 4, L)
+    slw = crand(rng, 5)
+    prop.hs_apply(ws, d4, slw)
+    out, lw, _ = ws.download()
+    tol = 1e-14 if precision == 0 else 2e-6
+    for i in range(5):
+        s = NiupNidnSample(L)
+        s.diag00, s.diag01, s.diag10, s.diag11 = d4[i]
+        s.logw = slw[i]
+        ref, rlw = niupnidn_apply_right(s, wfs[i], logw[i])
+        assert np.max(np.abs(out[i] - ref)) < tol * max(1.0, np.max(np.abs(ref)))
+        assert abs(lw[i] - rlw) < 1e-14

@@ -312,3 +312,36 @@ def test_driver_trajectory_matches_oracle(schedule):
             assert abs(x - y) < 1e-9 * abs(y), (x, y)
     if schedule == "adjust_then_growth":
         assert np.allclose(dev.model.bg, ref.model.choleskyBg, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_taylor_fixed_order_zero_freezes_the_first_walkers_order(precision):
+    """fixedOrder with taylorOrder 0 (logHop2isSD2isOperation.cpp:122-133): the first application
+    takes the dynamic order of its first walker (the first alive one: walker 0 is dead here) and
+    every walker of that and later applications takes exactly that many terms, as one reference
+    operation object applied walker by walker in index order does."""
+    bg = np.linspace(0.1, -0.1, 12)
+    om, dm, phiT, prop, ws, wfs, logw, method = make_case(precision=precision, bg=bg, dt=0.02,
+                                                          taylor="fixedOrder", taylor_order=0)
+    alive = np.ones(ws.nw, dtype=np.int32)
+    alive[0] = 0
+    ws.upload(wfs, logw, alive)
+    rng = np.random.default_rng(12)
+    G = dm.G
+    hs = om.returnExpMinusAlphaV(method.dt)
+    op = LogHop2isSD2isOperation("fixedOrder", taylorOrder=0)
+    tol = RTOL if precision == 0 else 1e-4
+    cur = [SD2is(wfs[i], dm.Nup, dm.Ndn, logw[i]) for i in range(ws.nw)]
+    for app in range(2):
+        scale = 1.0 + 3.0 * app  # larger fields on the second application: the frozen order still holds
+        aux = scale * (rng.normal(size=(ws.nw, G)) + 0.2j * rng.normal(size=(ws.nw, G)))
+        force = 0.3 * crand(rng, ws.nw, G)
+        prop.sample_apply(ws, aux, force)
+        orders = prop.taylor_orders(ws)
+        out, lw, al = ws.download()
+        for i in range(1, ws.nw):
+            s = hs.getTwoBodySampleFromAuxForce(aux[i], force[i])
+            cur[i] = op.applyToRight(s, cur[i])
+            assert orders[i] == op.currentOrder == op.taylorOrder, (app, i)
+            assert np.max(np.abs(out[i] - cur[i].wf)) <= tol * max(1.0, np.max(np.abs(cur[i].wf)))
+        assert np.array_equal(out[0], wfs[0])

@@ -5,6 +5,7 @@ import json
 import os
 
 import numpy as np
+import pytest
 
 from afqmclab_amd import reference_io as rio
 from afqmclab_amd.model import HubbardModel, spin_block_trial
@@ -68,3 +69,47 @@ def test_afqmc_param_written_and_searched_as_the_reference_does(tmp_path):
     with open(p, "a") as f:
         f.write("forceCap = 2.5\n")
     assert rio.search_value(p, "forceCap", float) == 2.5
+
+
+PH_EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "afqmclab_amd", "afqmc_ph_run")
+
+
+@pytest.mark.skipif(not os.path.exists(PH_EXE), reason="afqmc_ph_run not built (no HDF5 library at build time)")
+def test_real_materials_model_param_hdf5_roundtrip(tmp_path):
+    """RealMaterialMolecule's HDF5 model_param (realMaterialMolecule.cpp:41-94) written from raw
+    arrays and read back by the phaseless driver's reader: every dataset bit for bit; a dataset of
+    the wrong size is an error, as readFile's size check makes it (readWriteHdf5.cpp:42-48)."""
+    import subprocess
+    rng = np.random.default_rng(3)
+    L, Nup, Ndn, G = 7, 3, 2, 5
+    arrays = {"t": rng.normal(size=(L, L)), "K": rng.normal(size=(L, L)), "choleskyVecs": rng.normal(size=(L, L, G)),
+              "choleskyBg": rng.normal(size=G)}
+    (tmp_path / "model_dims").write_text(f"{L} {Nup} {Ndn} {G}\n")
+    for k, a in arrays.items():
+        np.asfortranarray(a).ravel(order="F").astype("<f8").tofile(str(tmp_path / (k + ".bin")))
+    r = subprocess.run([PH_EXE, "--write-model", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "model_param").read_bytes()[:8] == b"\x89HDF\r\n\x1a\n"
+    r = subprocess.run([PH_EXE, "--read-model", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "model_dims.read").read_text().split() == [str(L), str(Nup), str(Ndn), str(G)]
+    for k, a in arrays.items():
+        assert np.array_equal(np.fromfile(str(tmp_path / (k + ".read.bin"))), np.asfortranarray(a).ravel(order="F"))
+    # a model_dims that promises more Cholesky vectors than the file holds
+    (tmp_path / "model_dims").write_text(f"{L} {Nup} {Ndn} {G + 1}\n")
+    np.zeros((L, L, G + 1)).ravel().tofile(str(tmp_path / "choleskyVecs.bin"))
+    np.zeros(G + 1).tofile(str(tmp_path / "choleskyBg.bin"))
+    sub = tmp_path / "bad"
+    sub.mkdir()
+    r = subprocess.run([PH_EXE, "--write-model", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    import shutil
+    shutil.copy(str(tmp_path / "model_param"), str(sub / "model_param"))
+    # hand the reader a file whose choleskyNumber does not match its vectors
+    (sub / "model_dims").write_text(f"{L} {Nup} {Ndn} {G}\n")
+    for k, a in arrays.items():
+        np.asfortranarray(a).ravel(order="F").astype("<f8").tofile(str(sub / (k + ".bin")))
+    r = subprocess.run([PH_EXE, "--read-model", str(sub)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0  # a consistent file reads
+    r = subprocess.run([PH_EXE, "--read-model", str(tmp_path / "missing")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "HDF5" in r.stderr
