@@ -58,6 +58,13 @@ SIGNATURES = {
     "afqmc_graph_destroy": (C.c_int, [_vp]),
     "afqmc_cp_set_tiles": (C.c_int, [_vp, _ip]),
     "afqmc_cp_autotune": (C.c_int, [_vp, _vp, C.c_int, _ip, _dp]),
+    "afqmc_cp_group_create": (C.c_int, [C.c_int, _vp, _vp, _vp]),
+    "afqmc_cp_group_destroy": (C.c_int, [_vp]),
+    "afqmc_cp_group_set_tiles": (C.c_int, [_vp, _ip]),
+    "afqmc_cp_group_step": (C.c_int, [_vp]),
+    "afqmc_cp_group_stabilize": (C.c_int, [_vp]),
+    "afqmc_cp_group_pop_control": (C.c_int, [_vp]),
+    "afqmc_cp_group_table_count": (C.c_int, [_vp]),
     "afqmc_cp_create": (C.c_int, [_vp, C.POINTER(CPDesc), C.POINTER(_vp)]),
     "afqmc_cp_destroy": (C.c_int, [_vp]),
     "afqmc_cp_get_hs_constants": (C.c_int, [_vp, _dp]),

@@ -96,6 +96,50 @@ class Graph:
             pass
 
 
+class CPGroup:
+    """Several constraint-path runs (a propagator and one of its walker sets each, all on one
+    context) advanced together: step / stabilize / pop_control act on every member as the
+    propagator's own calls do, with each kernel of a phase launched once over all members."""
+
+    def __init__(self, members, tiles=None):
+        self.members = list(members)
+        n = len(self.members)
+        cps = (C.c_void_p * n)(*[p.h for p, _ in self.members])
+        ws = (C.c_void_p * n)(*[w.h for _, w in self.members])
+        h = C.c_void_p()
+        check(lib().afqmc_cp_group_create(n, cps, ws, C.byref(h)))
+        self.h = h
+        if tiles is not None:
+            self.set_tiles(*tiles)
+
+    def set_tiles(self, propagate=0, overlap=0, green=0):
+        t = np.array([propagate, overlap, green], dtype=np.int32)
+        check(lib().afqmc_cp_group_set_tiles(self.h, iptr(t)))
+
+    def step(self):
+        check(lib().afqmc_cp_group_step(self.h))
+
+    def stabilize(self):
+        check(lib().afqmc_cp_group_stabilize(self.h))
+
+    def pop_control(self):
+        check(lib().afqmc_cp_group_pop_control(self.h))
+
+    def table_count(self):
+        return lib().afqmc_cp_group_table_count(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().afqmc_cp_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def walkers_to_buffer(wfs):
     """List/array of 2L x N walkers -> (nw, N, 2L) C-contiguous buffer (each walker column-major)."""
     a = np.asarray(wfs, dtype=np.complex128)

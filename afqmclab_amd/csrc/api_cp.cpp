@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -181,12 +182,18 @@ int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
   // mixed path: QR in FP64 on a widened copy of the FP32 walkers
   hipStream_t s = cp->ctx->stream;
   int rc;
-  for (int b = 0; b < cp->nblk; ++b) {
-    if ((rc = launch_convert_f32_f64(w->wf[b].p, cp->qr64[b].p, w->wsize(b) * w->nw, s))) return rc;
+  {
+    GroupFork fork;
+    for (int b = 0; b < cp->nblk; ++b) {
+      fork.branch();
+      if ((rc = launch_convert_f32_f64(w->wf[b].p, cp->qr64[b].p, w->wsize(b) * w->nw, s))) return rc;
+    }
   }
   // the result is rounded to FP32, so a first-pass Q orthonormal to 1e-8 is final
   if ((rc = qr_impl_f64(cp, w, cp->qr64, cp->qr64b, 1e-8))) return rc;
+  GroupFork fork;
   for (int b = 0; b < cp->nblk; ++b) {
+    fork.branch();
     if ((rc = launch_convert_f64_f32(cp->qr64[b].p, w->wf[b].p, w->wsize(b) * w->nw, s))) return rc;
   }
   return 0;
@@ -1099,5 +1106,108 @@ int afqmc_cp_autotune(afqmc_cp cp, afqmc_walkers w, int reps, int* tiles, double
       for (int k = 0; k < GEMM_TILE_COUNT; ++k) ms[r * GEMM_TILE_COUNT + k] = t[r][k];
   return 0;
 }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ groups of runs
+struct afqmc_cp_group_s {
+  afqmc_ctx ctx = nullptr;
+  std::vector<afqmc_cp> cps;
+  std::vector<afqmc_walkers> ws;
+  int fp32 = 0;
+  int tile[3] = {GEMM_TILE_64x64, GEMM_TILE_64x64, GEMM_TILE_64x64};
+  GroupTables tables;
+};
+
+namespace {
+
+// per_member(cp, w) for every member with the launches recorded, then launched grouped.  The
+// members' own tile shapes give way to the group's for the call, so each GEMM phase is one launch.
+template <class F>
+int run_grouped(afqmc_cp_group g, F&& per_member) {
+  REQUIRE(!grouping(), "grouped calls do not nest");
+  ScopedDevice sd(g->ctx);
+  GroupCollector c;
+  c.tables = &g->tables;
+  const size_t n = g->cps.size();
+  std::vector<std::array<int, 3>> saved(n);
+  for (size_t i = 0; i < n; ++i)
+    for (int r = 0; r < 3; ++r) {
+      saved[i][r] = g->cps[i]->tile[r];
+      g->cps[i]->tile[r] = g->tile[r];
+    }
+  g_group = &c;
+  int rc = 0;
+  for (size_t i = 0; i < n && !rc; ++i) {
+    c.member = (int)i;
+    c.pos = 0;
+    rc = per_member(g->cps[i], g->ws[i]);
+  }
+  g_group = nullptr;
+  for (size_t i = 0; i < n; ++i)
+    for (int r = 0; r < 3; ++r) g->cps[i]->tile[r] = saved[i][r];
+  if (rc) return rc;
+  return group_flush_all(c, g->ctx->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int afqmc_cp_group_create(int n, const afqmc_cp* cps, const afqmc_walkers* walkers, afqmc_cp_group* out) {
+  REQUIRE(n >= 1 && cps && walkers && out, "invalid argument");
+  for (int i = 0; i < n; ++i) {
+    REQUIRE(cps[i] && walkers[i] && walkers[i]->cp == cps[i], "member %d: a propagator and one of its walker sets", i);
+    REQUIRE(cps[i]->ctx == cps[0]->ctx, "member %d: every member on the first member's context", i);
+    REQUIRE(cps[i]->fp32 == cps[0]->fp32, "member %d: every member in the same precision", i);
+    REQUIRE(!cps[i]->split_bf16, "member %d: the split-BF16 propagation has no grouped form", i);
+    for (int j = 0; j < i; ++j) REQUIRE(cps[j] != cps[i], "members %d and %d share a propagator", j, i);
+  }
+  afqmc_cp_group g = new afqmc_cp_group_s();
+  g->ctx = cps[0]->ctx;
+  g->cps.assign(cps, cps + n);
+  g->ws.assign(walkers, walkers + n);
+  g->fp32 = cps[0]->fp32;
+  *out = g;
+  return 0;
+}
+
+int afqmc_cp_group_destroy(afqmc_cp_group g) {
+  if (!g) return 0;
+  ScopedDevice sd(g->ctx);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+  return 0;
+}
+
+int afqmc_cp_group_set_tiles(afqmc_cp_group g, const int* tiles) {
+  REQUIRE(g && tiles, "null argument");
+  for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= 0 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
+  REQUIRE(gemm_tile_green(tiles[2]), "the Green GEMM takes tile shape 0, 1, 4, 5, 7, 8 or 9");
+  for (int r = 0; r < 3; ++r) g->tile[r] = tiles[r];
+  return 0;
+}
+
+int afqmc_cp_group_step(afqmc_cp_group g) {
+  REQUIRE(g, "null argument");
+  return run_grouped(g, [&](afqmc_cp cp, afqmc_walkers w) {
+    return g->fp32 ? step_impl<float>(cp, w, nullptr, nullptr) : step_impl<double>(cp, w, nullptr, nullptr);
+  });
+}
+
+int afqmc_cp_group_stabilize(afqmc_cp_group g) {
+  REQUIRE(g, "null argument");
+  return run_grouped(g, [&](afqmc_cp cp, afqmc_walkers w) { return stabilize_impl(cp, w); });
+}
+
+int afqmc_cp_group_pop_control(afqmc_cp_group g) {
+  REQUIRE(g, "null argument");
+  return run_grouped(g, [&](afqmc_cp cp, afqmc_walkers w) {
+    return g->fp32 ? pop_control_impl<float>(cp, w, nullptr, nullptr, nullptr)
+                   : pop_control_impl<double>(cp, w, nullptr, nullptr, nullptr);
+  });
+}
+
+int afqmc_cp_group_table_count(afqmc_cp_group g) { return g ? (int)g->tables.count() : -1; }
 
 }  // extern "C"

@@ -63,8 +63,17 @@ void afqmc_set_error(const char* fmt, ...);
     }                                                                              \
   } while (0)
 
+// the grouped-launch collector open on this thread (group.hpp), if any: a kernel launched
+// directly while it collects would run ahead of the recorded ones
+struct GroupCollector;
+extern thread_local GroupCollector* g_group;
+
 #define HIP_LAUNCH_CHECK()                                                         \
   do {                                                                             \
+    if (g_group) {                                                                 \
+      afqmc_set_error("%s:%d a kernel with no grouped form ran inside a grouped call", __FILE__, __LINE__); \
+      return -1;                                                                   \
+    }                                                                              \
     hipError_t _e = hipGetLastError();                                             \
     if (_e != hipSuccess) {                                                        \
       afqmc_set_error("%s:%d launch -> %s", __FILE__, __LINE__, hipGetErrorString(_e)); \

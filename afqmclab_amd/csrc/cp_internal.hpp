@@ -11,6 +11,7 @@
 #include "../../include/afqmc_mi355x.h"
 #include "cp_kernels.hpp"
 #include "gemm.hpp"
+#include "group.hpp"
 #include "runtime.hpp"
 #include "small_dense.hpp"
 
@@ -100,7 +101,9 @@ int cp_gemm(afqmc_cp cp, int opa, int epi, const GemmArgs& a) {
 // overlap O_b = phiT_b^H W_b for all walkers ("giant" GEMM with shared A)
 template <typename T>
 int overlap_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf) {
+  GroupFork fork;  // the blocks are independent: a grouped call launches them together
   for (int b = 0; b < cp->nblk; ++b) {
+    fork.branch();
     GemmArgs a = gemm_base<T>();
     a.M = cp->Nb[b]; a.N = cp->Nb[b] * w->nw; a.K = cp->Mb[b]; a.batch = 1;
     a.A = cp->phiT[b].p; a.lda = cp->Mb[b]; a.strideA = 0;
@@ -115,7 +118,9 @@ int overlap_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf) {
 
 template <typename T>
 int gj_blocks(afqmc_cp cp, afqmc_walkers w, bool write_inverse, const int* alive) {
+  GroupFork fork;  // the blocks are independent: a grouped call launches them together
   for (int b = 0; b < cp->nblk; ++b) {
+    fork.branch();
     SmallDenseArgs s;
     memset(&s, 0, sizeof(s));
     s.N = cp->Nb[b]; s.batch = w->nw;
@@ -132,7 +137,9 @@ int gj_blocks(afqmc_cp cp, afqmc_walkers w, bool write_inverse, const int* alive
 // X_b = W_b O_b^{-1} with green row-dot epilogue
 template <typename T>
 int green_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, const int* alive, bool kinetic) {
+  GroupFork fork;  // the blocks are independent: a grouped call launches them together
   for (int b = 0; b < cp->nblk; ++b) {
+    fork.branch();
     GemmArgs a = gemm_base<T>();
     a.M = cp->Mb[b]; a.N = cp->Nb[b]; a.K = cp->Nb[b]; a.batch = w->nw;
     a.A = wf[b].p; a.lda = cp->Mb[b]; a.strideA = w->wsize(b);
@@ -152,7 +159,9 @@ int green_gemm(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, const int* alive,
 // W'_b = B_b diag(s) W_b  (all walkers, one GEMM per block)
 template <typename T>
 int propagate_gemm(afqmc_cp cp, afqmc_walkers w, int which, DevBuf<char>* in, DevBuf<char>* out, bool scaled) {
+  GroupFork fork;  // the blocks are independent: a grouped call launches them together
   for (int b = 0; b < cp->nblk; ++b) {
+    fork.branch();
     GemmArgs a = gemm_base<T>();
     a.M = cp->Mb[b]; a.N = cp->Nb[b] * w->nw; a.K = cp->Mb[b]; a.batch = 1;
     a.A = cp->expK[which][b].p; a.lda = cp->Mb[b];

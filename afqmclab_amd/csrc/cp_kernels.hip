@@ -3,6 +3,8 @@
 // HubbardSOCMeasureFixedSDSD (:178-254) and the comb (populationControlConfiguration.cpp:24-101).
 #include "cp_kernels.hpp"
 
+#include "group.hpp"
+
 #include <algorithm>
 #include <type_traits>
 
@@ -33,8 +35,12 @@ __device__ zd block_sum_z(zd v, double* sh) {
 }
 
 // ------------------------------------------------------------------ logO
-__global__ void logo_kernel(LogOArgs a) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+struct LogoK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const LogOArgs& a);
+};
+__device__ __forceinline__ void LogoK::run(const BlockIdx3& bk, const LogOArgs& a) {
+  const int w = bk.x * blockDim.x + threadIdx.x;
   if (w >= a.nw) return;
   if (a.mode != LOGO_MEASURE && a.mode != LOGO_ONLY && !a.alive[w]) {
     if (a.mode == LOGO_WEIGHT) a.weight[w] = 0.0;
@@ -91,7 +97,22 @@ __global__ void logo_kernel(LogOArgs a) {
 }
 
 // ------------------------------------------------------------------ alive prefix
-__global__ __launch_bounds__(1024) void alive_scan_kernel(int nw, const int* alive, int* prefix, DevScalars* sc) {
+struct AliveScanArgs {
+  const int* alive;
+  int* prefix;
+  DevScalars* sc;
+  int nw, pad;
+};
+struct AliveScanK {
+  static constexpr int NT = 1024, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const AliveScanArgs& g);
+};
+__device__ __forceinline__ void AliveScanK::run(const BlockIdx3& bk, const AliveScanArgs& g) {
+  const int nw = g.nw;
+  const int* alive = g.alive;
+  int* prefix = g.prefix;
+  DevScalars* sc = g.sc;
+
   __shared__ int part[1024];
   const int tid = threadIdx.x;
   const int chunk = (nw + 1023) / 1024;
@@ -192,8 +213,13 @@ __device__ __forceinline__ void hs_site_matrix(int decomp, zd gx, const zd* cdia
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
-  const int w = blockIdx.x;
+struct HsSampleK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const HsSampleArgs& a);
+};
+template <typename T>
+__device__ __forceinline__ void HsSampleK<T>::run(const BlockIdx3& bk, const HsSampleArgs& a) {
+  const int w = bk.x;
   if (!a.alive[w]) return;
   __shared__ double sh[16];
   const int L = a.L;
@@ -266,7 +292,18 @@ __global__ __launch_bounds__(256) void hs_sample_kernel(HsSampleArgs a) {
   }
 }
 
-__global__ void advance_counter_kernel(DevScalars* sc, int per_walker) {
+struct AdvanceArgs {
+  DevScalars* sc;
+  int per_walker, pad;
+};
+struct AdvanceK {
+  static constexpr int NT = 64, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const AdvanceArgs& a);
+};
+__device__ __forceinline__ void AdvanceK::run(const BlockIdx3& bk, const AdvanceArgs& a) {
+  DevScalars* sc = a.sc;
+  const int per_walker = a.per_walker;
+
   sc->counter += (unsigned long long)per_walker * (unsigned long long)sc->alive_count;
   sc->walker_steps += (unsigned long long)sc->alive_count;
 }
@@ -277,19 +314,35 @@ __global__ void advance_counter_kernel(DevScalars* sc, int per_walker) {
 // (column chunk, walker): the walker's 4 x L site matrices are staged in LDS once per workgroup,
 // and each work item moves S consecutive sites of one column as 16-byte vectors (S = 2 complex
 // FP32 or 1 complex FP64; S = 1 with 8-byte FP32 accesses when L is odd).  No 64-bit division.
+template <typename T>
+struct HsApplyArgs {
+  const cplx<T>* in;
+  cplx<T>* out;
+  const cplx<T>* d;
+  int L, N, cpb, pad;
+};
 template <typename T, int S>
-__global__ __launch_bounds__(256) void hs_apply_2x2_kernel(int L, int N, int cpb, const cplx<T>* __restrict__ in,
-                                                           cplx<T>* __restrict__ out, const cplx<T>* __restrict__ d) {
+struct HsApplyK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const HsApplyArgs<T>& g);
+};
+template <typename T, int S>
+__device__ __forceinline__ void HsApplyK<T, S>::run(const BlockIdx3& bk, const HsApplyArgs<T>& g) {
+  const int L = g.L, N = g.N, cpb = g.cpb;
+  const cplx<T>* __restrict__ in = g.in;
+  cplx<T>* __restrict__ out = g.out;
+  const cplx<T>* __restrict__ d = g.d;
+
   typedef cplx<T> ct;
   struct alignas(S * sizeof(ct)) vt { ct z[S]; };
   extern __shared__ __align__(16) unsigned char hs_lds[];
   vt* sd = reinterpret_cast<vt*>(hs_lds);
-  const int w = blockIdx.y, tid = threadIdx.x;
+  const int w = bk.y, tid = threadIdx.x;
   const int nv = L / S;  // site vectors per half column
   const vt* dw = reinterpret_cast<const vt*>(d + (long long)w * 4 * L);
   for (int k = tid; k < 4 * nv; k += blockDim.x) sd[k] = dw[k];
   __syncthreads();
-  const int j0 = blockIdx.x * cpb, j1 = min(N, j0 + cpb);
+  const int j0 = bk.x * cpb, j1 = min(N, j0 + cpb);
   const long long base = (long long)w * 2 * L * N;
   auto item = [&](int j, int v) {
     const long long o = base + (long long)j * 2 * L + (long long)v * S;
@@ -319,9 +372,29 @@ __global__ __launch_bounds__(256) void hs_apply_2x2_kernel(int L, int N, int cpb
 // ------------------------------------------------------------------ comb
 // popConfiguration (populationControlConfiguration.cpp:24-101): sequential
 // left-to-right sums in one thread, so bin edges round exactly as on the host.
-__global__ void comb_kernel(int L, const double* weight, const double* ubuf, DevScalars* sc, int* table,
-                            double* prob, int* number) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+struct CombArgs {
+  const double* weight;
+  const double* ubuf;
+  DevScalars* sc;
+  int* table;
+  double* prob;
+  int* number;
+  int L, pad;
+};
+struct CombK {
+  static constexpr int NT = 64, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const CombArgs& g);
+};
+__device__ __forceinline__ void CombK::run(const BlockIdx3& bk, const CombArgs& g) {
+  const int L = g.L;
+  const double* weight = g.weight;
+  const double* ubuf = g.ubuf;
+  DevScalars* sc = g.sc;
+  int* table = g.table;
+  double* prob = g.prob;
+  int* number = g.number;
+
+  if (threadIdx.x != 0) return;
   const unsigned long long key = sc->key;
   const unsigned long long c0 = sc->counter;
   double sum = 0.0, maxW = 0.0, minW = 1e20;
@@ -372,8 +445,19 @@ __global__ void comb_kernel(int L, const double* weight, const double* ubuf, Dev
 // (i + u_i) / L increase with i and the cumulative sums do not decrease, so that bin is the first j
 // over the whole array where the predicate holds, found by binary search (clamped at L-1 as the scan).
 constexpr int COMB_LDS_MAX = 4096;
-__global__ __launch_bounds__(1024) void comb_lds_kernel(int L, const double* weight, const double* ubuf, DevScalars* sc,
-                                                        int* table, double* prob, int* number) {
+struct CombLdsK {
+  static constexpr int NT = 1024, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const CombArgs& g);
+};
+__device__ __forceinline__ void CombLdsK::run(const BlockIdx3& bk, const CombArgs& g) {
+  const int L = g.L;
+  const double* weight = g.weight;
+  const double* ubuf = g.ubuf;
+  DevScalars* sc = g.sc;
+  int* table = g.table;
+  double* prob = g.prob;
+  int* number = g.number;
+
   __shared__ double p[COMB_LDS_MAX];
   __shared__ int num[COMB_LDS_MAX];
   __shared__ double s_sum;
@@ -445,19 +529,40 @@ __global__ __launch_bounds__(1024) void comb_lds_kernel(int L, const double* wei
 }
 
 template <typename T>
-__global__ void apply_table_kernel(int nw, cplx<T>* wf0, cplx<T>* wf1, long long ws0, long long ws1, zd* logw,
-                                   int* alive, const int* table) {
-  const int i = blockIdx.y;
+struct ApplyTableArgs {
+  cplx<T>* wf0;
+  cplx<T>* wf1;
+  long long ws0, ws1;
+  zd* logw;
+  int* alive;
+  const int* table;
+  int nw, pad;
+};
+template <typename T>
+struct ApplyTableK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const ApplyTableArgs<T>& g);
+};
+template <typename T>
+__device__ __forceinline__ void ApplyTableK<T>::run(const BlockIdx3& bk, const ApplyTableArgs<T>& g) {
+  cplx<T>* wf0 = g.wf0;
+  cplx<T>* wf1 = g.wf1;
+  const long long ws0 = g.ws0, ws1 = g.ws1;
+  zd* logw = g.logw;
+  int* alive = g.alive;
+  const int* table = g.table;
+
+  const int i = bk.y;
   const int src = table[i];
   if (src != i && src >= 0) {
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ws0; e += (long long)gridDim.x * blockDim.x)
+    for (long long e = (long long)bk.x * blockDim.x + threadIdx.x; e < ws0; e += (long long)bk.gx * blockDim.x)
       wf0[(long long)i * ws0 + e] = wf0[(long long)src * ws0 + e];
     if (wf1)
-      for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ws1; e += (long long)gridDim.x * blockDim.x)
+      for (long long e = (long long)bk.x * blockDim.x + threadIdx.x; e < ws1; e += (long long)bk.gx * blockDim.x)
         wf1[(long long)i * ws1 + e] = wf1[(long long)src * ws1 + e];
-    if (blockIdx.x == 0 && threadIdx.x == 0) logw[i] = logw[src];
+    if (bk.x == 0 && threadIdx.x == 0) logw[i] = logw[src];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) alive[i] = 1;
+  if (bk.x == 0 && threadIdx.x == 0) alive[i] = 1;
 }
 
 // ------------------------------------------------------------------ energies
@@ -562,8 +667,25 @@ __global__ void site_final_kernel(EnergyArgs a) {
 }
 
 // need[w] = alive && max_ij |S_ij - delta_ij| > tol (S = Q^H Q of walker w), skip[w] = alive && !need
-__global__ void ortho_check_kernel(int N, const zd* S, const int* alive, double tol, int* need) {
-  const int w = blockIdx.x;
+struct OrthoArgs {
+  const zd* S;
+  const int* alive;
+  int* need;
+  double tol;
+  int N, pad;
+};
+struct OrthoK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const OrthoArgs& g);
+};
+__device__ __forceinline__ void OrthoK::run(const BlockIdx3& bk, const OrthoArgs& g) {
+  const int N = g.N;
+  const zd* S = g.S;
+  const int* alive = g.alive;
+  const double tol = g.tol;
+  int* need = g.need;
+
+  const int w = bk.x;
   __shared__ double sh[8];
   const zd* s = S + (long long)w * N * N;
   double m = 0.0;
@@ -587,14 +709,44 @@ __global__ void ortho_check_kernel(int N, const zd* S, const int* alive, double 
   }
 }
 
-__global__ void qr_flag_kernel(int nw, const int* alive, const int* info, int* flag) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+struct QrFlagArgs {
+  const int* alive;
+  const int* info;
+  int* flag;
+  int nw, pad;
+};
+struct QrFlagK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const QrFlagArgs& g);
+};
+__device__ __forceinline__ void QrFlagK::run(const BlockIdx3& bk, const QrFlagArgs& g) {
+  const int nw = g.nw;
+  const int* alive = g.alive;
+  const int* info = g.info;
+  int* flag = g.flag;
+
+  const int w = bk.x * blockDim.x + threadIdx.x;
   if (w < nw) flag[w] = (alive[w] && info[w]) ? 1 : 0;
 }
 
 // S_w += c tr(S_w) I for flagged walkers: one workgroup per walker
-__global__ void qr_shift_kernel(int N, zd* S, const int* flag, double c) {
-  const int w = blockIdx.x;
+struct QrShiftArgs {
+  zd* S;
+  const int* flag;
+  double c;
+  int N, pad;
+};
+struct QrShiftK {
+  static constexpr int NT = 64, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const QrShiftArgs& g);
+};
+__device__ __forceinline__ void QrShiftK::run(const BlockIdx3& bk, const QrShiftArgs& g) {
+  const int N = g.N;
+  zd* S = g.S;
+  const int* flag = g.flag;
+  const double c = g.c;
+
+  const int w = bk.x;
   if (!flag[w]) return;
   zd* s = S + (long long)w * N * N;
   __shared__ double sh[4];
@@ -607,8 +759,26 @@ __global__ void qr_shift_kernel(int N, zd* S, const int* flag, double c) {
 
 // one workgroup: flagged walkers count as shifted, those whose shifted Cholesky broke down as well
 // die (their Gram matrix is singular: a rank-deficient walker); the counts are added by one thread
-__global__ void qr_kill_kernel(int nw, int* flag, int* flag2, const int* info, int* alive, DevScalars* sc,
-                               int count_shifted) {
+struct QrKillArgs {
+  int* flag;
+  int* flag2;
+  const int* info;
+  int* alive;
+  DevScalars* sc;
+  int nw, count_shifted;
+};
+struct QrKillK {
+  static constexpr int NT = 1024, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const QrKillArgs& g);
+};
+__device__ __forceinline__ void QrKillK::run(const BlockIdx3& bk, const QrKillArgs& g) {
+  const int nw = g.nw, count_shifted = g.count_shifted;
+  int* flag = g.flag;
+  int* flag2 = g.flag2;
+  const int* info = g.info;
+  int* alive = g.alive;
+  DevScalars* sc = g.sc;
+
   __shared__ double sh[16];
   double nshift = 0.0, nkill = 0.0;
   for (int w = threadIdx.x; w < nw; w += blockDim.x) {
@@ -634,9 +804,27 @@ __global__ void set_et_kernel(DevScalars* sc) {
   sc->ET = e.re;
 }
 
-__global__ void logw_add_kernel(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, double m0,
-                                double m1) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+struct LogwAddArgs {
+  zd* logw;
+  const int* alive;
+  const zd* add0;
+  const zd* add1;
+  double m0, m1;
+  int nw, pad;
+};
+struct LogwAddK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const LogwAddArgs& g);
+};
+__device__ __forceinline__ void LogwAddK::run(const BlockIdx3& bk, const LogwAddArgs& g) {
+  const int nw = g.nw;
+  zd* logw = g.logw;
+  const int* alive = g.alive;
+  const zd* add0 = g.add0;
+  const zd* add1 = g.add1;
+  const double m0 = g.m0, m1 = g.m1;
+
+  const int w = bk.x * blockDim.x + threadIdx.x;
   if (w >= nw || (alive && !alive[w])) return;
   double d = m0 * add0[w].re + m1 * add0[w].im;
   if (add1) d += m0 * add1[w].re + m1 * add1[w].im;
@@ -844,28 +1032,26 @@ __global__ void den_sum_kernel(int nw, const zd* den, const int* alive, zd* acc)
 
 int launch_logo(const LogOArgs& a, hipStream_t s) {
   if (a.nw <= 0) return 0;
-  hipLaunchKernelGGL(logo_kernel, dim3((a.nw + 255) / 256), dim3(256), 0, s, a);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  return glaunch<LogoK>(a, dim3((a.nw + 255) / 256), dim3(256), 0, s);
 }
 
 int launch_alive_scan(int nw, const int* alive, int* prefix, DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(alive_scan_kernel, dim3(1), dim3(1024), 0, s, nw, alive, prefix, sc);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  AliveScanArgs g;
+  memset(&g, 0, sizeof(g));
+  g.alive = alive; g.prefix = prefix; g.sc = sc; g.nw = nw;
+  return glaunch<AliveScanK>(g, dim3(1), dim3(1024), 0, s);
 }
 
 int launch_hs_sample(const HsSampleArgs& a, hipStream_t s) {
-  if (a.fp32) hipLaunchKernelGGL(hs_sample_kernel<float>, dim3(a.nw), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(hs_sample_kernel<double>, dim3(a.nw), dim3(256), 0, s, a);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  if (a.fp32) return glaunch<HsSampleK<float>>(a, dim3(a.nw), dim3(256), 0, s);
+  return glaunch<HsSampleK<double>>(a, dim3(a.nw), dim3(256), 0, s);
 }
 
 int launch_advance_counter(DevScalars* sc, int per_walker, hipStream_t s) {
-  hipLaunchKernelGGL(advance_counter_kernel, dim3(1), dim3(1), 0, s, sc, per_walker);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  AdvanceArgs g;
+  memset(&g, 0, sizeof(g));
+  g.sc = sc; g.per_walker = per_walker;
+  return glaunch<AdvanceK>(g, dim3(1), dim3(1), 0, s);
 }
 
 template <typename T>
@@ -879,40 +1065,39 @@ int launch_hs_apply_2x2(int nw, int L, int N, const void* in, void* out, const v
   const int nv = L / S;
   const int cpb = std::max(1, std::min(N, (64 * 256) / std::max(nv, 1)));
   dim3 grid((unsigned)((N + cpb - 1) / cpb), (unsigned)nw);
-  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in);
-  const cplx<T>* dd = reinterpret_cast<const cplx<T>*>(d2x2);
-  cplx<T>* dst = reinterpret_cast<cplx<T>*>(out);
+  HsApplyArgs<T> g;
+  memset(&g, 0, sizeof(g));
+  g.in = reinterpret_cast<const cplx<T>*>(in);
+  g.d = reinterpret_cast<const cplx<T>*>(d2x2);
+  g.out = reinterpret_cast<cplx<T>*>(out);
+  g.L = L; g.N = N; g.cpb = cpb;
   if constexpr (std::is_same<T, float>::value) {
-    if (pair) hipLaunchKernelGGL((hs_apply_2x2_kernel<T, 2>), grid, dim3(256), lds, s, L, N, cpb, src, dst, dd);
-    else hipLaunchKernelGGL((hs_apply_2x2_kernel<T, 1>), grid, dim3(256), lds, s, L, N, cpb, src, dst, dd);
-  } else {
-    hipLaunchKernelGGL((hs_apply_2x2_kernel<T, 1>), grid, dim3(256), lds, s, L, N, cpb, src, dst, dd);
+    if (pair) return glaunch<HsApplyK<T, 2>>(g, grid, dim3(256), lds, s);
   }
-  HIP_LAUNCH_CHECK();
-  return 0;
+  return glaunch<HsApplyK<T, 1>>(g, grid, dim3(256), lds, s);
 }
 template int launch_hs_apply_2x2<double>(int, int, int, const void*, void*, const void*, hipStream_t);
 template int launch_hs_apply_2x2<float>(int, int, int, const void*, void*, const void*, hipStream_t);
 
 int launch_comb(int nw, const double* weight, const double* ubuf, DevScalars* sc, int* table, double* prob,
                 int* number, hipStream_t s) {
-  if (nw <= COMB_LDS_MAX)
-    hipLaunchKernelGGL(comb_lds_kernel, dim3(1), dim3(1024), 0, s, nw, weight, ubuf, sc, table, prob, number);
-  else
-    hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(64), 0, s, nw, weight, ubuf, sc, table, prob, number);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  CombArgs g;
+  memset(&g, 0, sizeof(g));
+  g.weight = weight; g.ubuf = ubuf; g.sc = sc; g.table = table; g.prob = prob; g.number = number; g.L = nw;
+  if (nw <= COMB_LDS_MAX) return glaunch<CombLdsK>(g, dim3(1), dim3(1024), 0, s);
+  return glaunch<CombK>(g, dim3(1), dim3(64), 0, s);
 }
 
 template <typename T>
 int launch_apply_table(int nw, int nblk, void* const* wf, const long long* wsize, zd* logw, int* alive,
                        const int* table, hipStream_t s) {
-  dim3 grid(8, nw);
-  hipLaunchKernelGGL(apply_table_kernel<T>, grid, dim3(256), 0, s, nw, reinterpret_cast<cplx<T>*>(wf[0]),
-                     nblk > 1 ? reinterpret_cast<cplx<T>*>(wf[1]) : nullptr, wsize[0], nblk > 1 ? wsize[1] : 0, logw,
-                     alive, table);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  ApplyTableArgs<T> g;
+  memset(&g, 0, sizeof(g));
+  g.wf0 = reinterpret_cast<cplx<T>*>(wf[0]);
+  g.wf1 = nblk > 1 ? reinterpret_cast<cplx<T>*>(wf[1]) : nullptr;
+  g.ws0 = wsize[0]; g.ws1 = nblk > 1 ? wsize[1] : 0;
+  g.logw = logw; g.alive = alive; g.table = table; g.nw = nw;
+  return glaunch<ApplyTableK<T>>(g, dim3(8, nw), dim3(256), 0, s);
 }
 template int launch_apply_table<double>(int, int, void* const*, const long long*, zd*, int*, const int*, hipStream_t);
 template int launch_apply_table<float>(int, int, void* const*, const long long*, zd*, int*, const int*, hipStream_t);
@@ -931,32 +1116,36 @@ int launch_energy(const EnergyArgs& a, hipStream_t s) {
 
 int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, hipStream_t s) {
   if (nw <= 0) return 0;
-  hipLaunchKernelGGL(ortho_check_kernel, dim3(nw), dim3(256), 0, s, N, S, alive, tol, need);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  OrthoArgs g;
+  memset(&g, 0, sizeof(g));
+  g.S = S; g.alive = alive; g.need = need; g.tol = tol; g.N = N;
+  return glaunch<OrthoK>(g, dim3(nw), dim3(256), 0, s);
 }
 
 int launch_qr_flag(int nw, const int* alive, const int* info, int* flag, hipStream_t s) {
   if (nw <= 0) return 0;
-  hipLaunchKernelGGL(qr_flag_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, nw, alive, info, flag);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  QrFlagArgs g;
+  memset(&g, 0, sizeof(g));
+  g.alive = alive; g.info = info; g.flag = flag; g.nw = nw;
+  return glaunch<QrFlagK>(g, dim3((nw + 255) / 256), dim3(256), 0, s);
 }
 
 int launch_qr_shift(int nw, int N, zd* S, const int* flag, double c, hipStream_t s) {
   if (nw <= 0) return 0;
-  hipLaunchKernelGGL(qr_shift_kernel, dim3(nw), dim3(64), 0, s, N, S, flag, c);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  QrShiftArgs g;
+  memset(&g, 0, sizeof(g));
+  g.S = S; g.flag = flag; g.c = c; g.N = N;
+  return glaunch<QrShiftK>(g, dim3(nw), dim3(64), 0, s);
 }
 
 int launch_qr_kill(int nw, int* flag, int* flag2, const int* info, int* alive, DevScalars* sc,
                    int count_shifted, hipStream_t s) {
   if (nw <= 0) return 0;
-  hipLaunchKernelGGL(qr_kill_kernel, dim3(1), dim3(1024), 0, s, nw, flag, flag2, info, alive, sc,
-                     count_shifted);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  QrKillArgs g;
+  memset(&g, 0, sizeof(g));
+  g.flag = flag; g.flag2 = flag2; g.info = info; g.alive = alive; g.sc = sc; g.nw = nw;
+  g.count_shifted = count_shifted;
+  return glaunch<QrKillK>(g, dim3(1), dim3(1024), 0, s);
 }
 
 int launch_set_et_from_energy(DevScalars* sc, hipStream_t s) {
@@ -967,10 +1156,10 @@ int launch_set_et_from_energy(DevScalars* sc, hipStream_t s) {
 
 int launch_logw_add_real(int nw, zd* logw, const int* alive, const zd* add0, const zd* add1, int /*unused*/,
                          double mult0, double mult1, hipStream_t s) {
-  hipLaunchKernelGGL(logw_add_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, nw, logw, alive, add0, add1, mult0,
-                     mult1);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  LogwAddArgs g;
+  memset(&g, 0, sizeof(g));
+  g.logw = logw; g.alive = alive; g.add0 = add0; g.add1 = add1; g.m0 = mult0; g.m1 = mult1; g.nw = nw;
+  return glaunch<LogwAddK>(g, dim3((nw + 255) / 256), dim3(256), 0, s);
 }
 
 int launch_bp_field(const BpFieldArgs& a, hipStream_t s) {

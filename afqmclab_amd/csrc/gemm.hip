@@ -13,6 +13,8 @@
 // prefetched into registers while the current one is multiplied.
 #include "gemm.hpp"
 
+#include "group.hpp"
+
 #include <type_traits>
 
 namespace {
@@ -23,9 +25,13 @@ namespace {
 // matters because the K loop of a Green tile is only N/16 long.  The larger wave tiles already
 // sit at 256 registers.
 template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
-__global__ __launch_bounds__(WM * WN * 64)
-__attribute__((amdgpu_waves_per_eu((EPI == EPI_GREEN && (BM / WM / 16) * (BN / WN / 16) <= 4) ? 2 : 1)))
-void cgemm_kernel(GemmArgs p) {
+struct CgemmK {
+  static constexpr int NT = WM * WN * 64;
+  static constexpr int WPE = (EPI == EPI_GREEN && (BM / WM / 16) * (BN / WN / 16) <= 4) ? 2 : 1;
+  __device__ static void run(const BlockIdx3& bk, const GemmArgs& p);
+};
+template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
+__device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::run(const BlockIdx3& bk, const GemmArgs& p) {
   constexpr int NT = WM * WN * 64;
   constexpr int TM = BM / WM, TN = BN / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
@@ -44,9 +50,9 @@ void cgemm_kernel(GemmArgs p) {
   // workgroups that share an operand panel then run on the same XCD and read it through one L2.
   int bx, by, b;
   {
-    const unsigned long long gx = gridDim.x, gy = gridDim.y;
-    const unsigned long long total = gx * gy * gridDim.z;
-    const unsigned long long L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned long long gx = bk.gx, gy = bk.gy;
+    const unsigned long long total = gx * gy * bk.gz;
+    const unsigned long long L = bk.x + gx * (bk.y + gy * bk.z);
     const unsigned long long per = total / 8;
     const unsigned long long lg = (L < per * 8) ? (L % 8) * per + L / 8 : L;
     if (p.n_fastest) {
@@ -590,9 +596,7 @@ int run(const GemmArgs& a, hipStream_t s) {
     afqmc_set_error("cgemm grid too large (%u, %u, %u)", grid.x, grid.y, grid.z);
     return -1;
   }
-  hipLaunchKernelGGL((cgemm_kernel<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>), grid, dim3(WM * WN * 64), 0, s, a);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  return glaunch<CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>>(a, grid, dim3(WM * WN * 64), 0, s);
 }
 
 template <typename T, int OPA, int EPI, bool SCALE>

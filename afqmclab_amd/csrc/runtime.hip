@@ -1,8 +1,12 @@
 // Context management, error reporting and elementwise helper kernels.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <functional>
 
 #include "runtime.hpp"
+
+#include "group.hpp"
 
 static thread_local char g_err[1024] = "";
 
@@ -13,11 +17,77 @@ void afqmc_set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
+thread_local GroupCollector* g_group = nullptr;
+
+GroupTables::~GroupTables() {
+  for (auto& kv : map_) (void)hipFree(kv.second);
+  if (side_) (void)hipStreamDestroy(side_);
+}
+
+const void* GroupTables::get(const void* p, size_t n) {
+  std::string key(reinterpret_cast<const char*>(p), n);
+  auto it = map_.find(key);
+  if (it != map_.end()) return it->second;
+  // a stream of its own (not the capturing one) for the upload, so the call is legal while a
+  // graph capture in relaxed mode is open
+  if (!side_ && hipStreamCreateWithFlags(&side_, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  void* d = nullptr;
+  if (hipMalloc(&d, n) != hipSuccess) return nullptr;
+  if (hipMemcpyAsync(d, p, n, hipMemcpyHostToDevice, side_) != hipSuccess || hipStreamSynchronize(side_) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  map_.emplace(std::move(key), d);
+  return d;
+}
+
+int group_flush_all(GroupCollector& c, hipStream_t s) {
+  std::vector<GroupRecord> recs = std::move(c.recs);
+  c.recs.clear();
+  // by position, then kernel and block shape, then member order
+  std::vector<const GroupRecord*> order;
+  order.reserve(recs.size());
+  for (const GroupRecord& r : recs) order.push_back(&r);
+  auto shape = [](const GroupRecord* r) {
+    return ((unsigned long long)r->block.x << 40) ^ ((unsigned long long)r->block.y << 20) ^ r->block.z;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](const GroupRecord* a, const GroupRecord* b) {
+    if (a->pos != b->pos) return a->pos < b->pos;
+    if (a->key != b->key) return std::less<const void*>()(a->key, b->key);
+    if (shape(a) != shape(b)) return shape(a) < shape(b);
+    return a->seq < b->seq;
+  });
+  for (size_t i = 0; i < order.size();) {
+    size_t j = i + 1;
+    while (j < order.size() && order[j]->pos == order[i]->pos && order[j]->key == order[i]->key &&
+           shape(order[j]) == shape(order[i]))
+      ++j;
+    int rc = order[i]->flush(order.data() + i, (int)(j - i), s, c.tables);
+    if (rc) return rc;
+    i = j;
+  }
+  return 0;
+}
+
 namespace {
 
 template <typename A, typename B>
-__global__ void convert_kernel(const cplx<A>* in, cplx<B>* out, long long n) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+struct ConvertArgs {
+  const cplx<A>* in;
+  cplx<B>* out;
+  long long n;
+};
+template <typename A, typename B>
+struct ConvertK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const ConvertArgs<A, B>& g);
+};
+template <typename A, typename B>
+__device__ __forceinline__ void ConvertK<A, B>::run(const BlockIdx3& bk, const ConvertArgs<A, B>& g) {
+  const cplx<A>* in = g.in;
+  cplx<B>* out = g.out;
+  const long long n = g.n;
+  for (long long i = (long long)bk.x * blockDim.x + threadIdx.x; i < n; i += (long long)bk.gx * blockDim.x)
     out[i] = cmk<B>((B)in[i].re, (B)in[i].im);
 }
 
@@ -33,17 +103,25 @@ __global__ void logw_vec_add_kernel(int nw, zd* logw, const int* alive, const zd
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w < nw && alive[w]) logw[w] = cadd(logw[w], v[w]);
 }
-__global__ void select_alive_kernel(int nw, long long n16, const int* alive, const uint4* src, uint4* dst) {
-  const int w = blockIdx.y;
-  if (!alive[w]) return;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
-    dst[(long long)w * n16 + i] = src[(long long)w * n16 + i];
-}
-__global__ void select_alive_kernel8(int nw, long long n8, const int* alive, const uint2* src, uint2* dst) {
-  const int w = blockIdx.y;
-  if (!alive[w]) return;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += (long long)gridDim.x * blockDim.x)
-    dst[(long long)w * n8 + i] = src[(long long)w * n8 + i];
+template <typename E>
+struct SelectArgs {
+  const int* alive;
+  const E* src;
+  E* dst;
+  long long n;
+};
+template <typename E>
+struct SelectK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const SelectArgs<E>& g);
+};
+template <typename E>
+__device__ __forceinline__ void SelectK<E>::run(const BlockIdx3& bk, const SelectArgs<E>& g) {
+  const long long n = g.n;
+  const int w = bk.y;
+  if (!g.alive[w]) return;
+  for (long long i = (long long)bk.x * blockDim.x + threadIdx.x; i < n; i += (long long)bk.gx * blockDim.x)
+    g.dst[(long long)w * n + i] = g.src[(long long)w * n + i];
 }
 __global__ void reset_measure_kernel(DevScalars* sc, zd* site_acc, int L) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,16 +138,14 @@ __global__ void reset_measure_kernel(DevScalars* sc, zd* site_acc, int L) {
 int launch_convert_f32_f64(const void* in, void* out, long long n, hipStream_t s) {
   unsigned g = (unsigned)std::min<long long>((n + 255) / 256, 65536);
   if (n <= 0) return 0;
-  hipLaunchKernelGGL((convert_kernel<float, double>), dim3(g), dim3(256), 0, s, (const zf*)in, (zd*)out, n);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  ConvertArgs<float, double> a{(const zf*)in, (zd*)out, n};
+  return glaunch<ConvertK<float, double>>(a, dim3(g), dim3(256), 0, s);
 }
 int launch_convert_f64_f32(const void* in, void* out, long long n, hipStream_t s) {
   unsigned g = (unsigned)std::min<long long>((n + 255) / 256, 65536);
   if (n <= 0) return 0;
-  hipLaunchKernelGGL((convert_kernel<double, float>), dim3(g), dim3(256), 0, s, (const zd*)in, (zf*)out, n);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  ConvertArgs<double, float> a{(const zd*)in, (zf*)out, n};
+  return glaunch<ConvertK<double, float>>(a, dim3(g), dim3(256), 0, s);
 }
 int launch_logw_shift(int nw, const zd* logw, zd* out, zd c, hipStream_t s) {
   hipLaunchKernelGGL(logw_shift_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, nw, logw, out, c);
@@ -87,17 +163,14 @@ int launch_logw_vec_add(int nw, zd* logw, const int* alive, const zd* v, hipStre
   return 0;
 }
 int launch_select_alive(int nw, long long n, size_t esz, const int* alive, const void* src, void* dst, hipStream_t s) {
+  if (nw <= 0 || n <= 0) return 0;
+  dim3 grid((unsigned)std::min<long long>((n + 255) / 256, 1024), nw);
   if (esz == 16) {
-    long long n16 = n;
-    dim3 grid((unsigned)std::min<long long>((n16 + 255) / 256, 1024), nw);
-    hipLaunchKernelGGL(select_alive_kernel, grid, dim3(256), 0, s, nw, n16, alive, (const uint4*)src, (uint4*)dst);
-  } else {
-    long long n8 = n;
-    dim3 grid((unsigned)std::min<long long>((n8 + 255) / 256, 1024), nw);
-    hipLaunchKernelGGL(select_alive_kernel8, grid, dim3(256), 0, s, nw, n8, alive, (const uint2*)src, (uint2*)dst);
+    SelectArgs<uint4> g{alive, (const uint4*)src, (uint4*)dst, n};
+    return glaunch<SelectK<uint4>>(g, grid, dim3(256), 0, s);
   }
-  HIP_LAUNCH_CHECK();
-  return 0;
+  SelectArgs<uint2> g{alive, (const uint2*)src, (uint2*)dst, n};
+  return glaunch<SelectK<uint2>>(g, grid, dim3(256), 0, s);
 }
 int launch_reset_measure(DevScalars* sc, zd* site_acc, int L, hipStream_t s) {
   hipLaunchKernelGGL(reset_measure_kernel, dim3((4 * L + 255) / 256 + (L == 0)), dim3(256), 0, s, sc, site_acc, L);

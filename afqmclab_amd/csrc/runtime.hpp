@@ -8,6 +8,7 @@
 #include "../../include/afqmc_mi355x.h"
 #include "common.hpp"
 #include "cp_kernels.hpp"
+#include "group.hpp"
 
 // Optional per-region kernel timing: when enabled, tagged launches are bracketed
 // by HIP events on the context stream; totals are resolved on query.
@@ -63,7 +64,7 @@ struct RegionTimer {
   int reg;
   hipEvent_t a = nullptr, b = nullptr;
   RegionTimer(afqmc_ctx c, int r) : ctx(c), reg(r) {
-    if (ctx && ctx->timing && !capturing()) {
+    if (ctx && ctx->timing && !capturing() && !grouping()) {
       // a region whose events cannot be made or recorded is left out of the timing
       if (hipEventCreate(&a) != hipSuccess) a = nullptr;
       if (a && hipEventCreate(&b) != hipSuccess) b = nullptr;

@@ -33,6 +33,8 @@
 //   the Gauss-Jordan kernel.
 #include "small_dense.hpp"
 
+#include "group.hpp"
+
 #include <type_traits>
 #include <utility>
 
@@ -120,8 +122,13 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 // others in LDS (gl[wave][c - NL][lane], lanes below LL): all of them for 129..160 rows, the
 // second half for the 8-wave kernel of 97..112 rows, none otherwise.
 template <typename T, int NR, int NC, int WV, int NL>
-__global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(4))) void gj_kernel(SmallDenseArgs p) {
-  const int bidx = blockIdx.x;
+struct GjK {
+  static constexpr int NT = 64 * WV, WPE = 4;
+  __device__ static void run(const BlockIdx3& bk, const SmallDenseArgs& p);
+};
+template <typename T, int NR, int NC, int WV, int NL>
+__device__ __forceinline__ void GjK<T, NR, NC, WV, NL>::run(const BlockIdx3& bk, const SmallDenseArgs& p) {
+  const int bidx = bk.x;
   if (p.alive && !p.alive[bidx / p.alive_div]) return;
   const int N = p.N;
   const int lane = threadIdx.x & 63;
@@ -345,8 +352,13 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(4))) vo
 
 // Cholesky and R^{-1}, row layout: y[c][r] = element (wave + 16c, lane + 64r).
 template <typename T, int NR, int NC>
-__global__ __launch_bounds__(64 * WAVES) void chol_kernel(SmallDenseArgs p) {
-  const int bidx = blockIdx.x;
+struct CholK {
+  static constexpr int NT = 64 * WAVES, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const SmallDenseArgs& p);
+};
+template <typename T, int NR, int NC>
+__device__ __forceinline__ void CholK<T, NR, NC>::run(const BlockIdx3& bk, const SmallDenseArgs& p) {
+  const int bidx = bk.x;
   if (p.alive && !p.alive[bidx / p.alive_div]) return;
   const int N = p.N;
   const int lane = threadIdx.x & 63;
@@ -561,15 +573,11 @@ int cu_count() {
 
 template <typename T, int NR, int NC, int WV = WAVES, int NL = NC>
 int run_gj(const SmallDenseArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((gj_kernel<T, NR, NC, WV, NL>), dim3(a.batch), dim3(64 * WV), 0, s, a);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  return glaunch<GjK<T, NR, NC, WV, NL>>(a, dim3(a.batch), dim3(64 * WV), 0, s);
 }
 template <typename T, int NR, int NC>
 int run_chol(const SmallDenseArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((chol_kernel<T, NR, NC>), dim3(a.batch), dim3(64 * WAVES), 0, s, a);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  return glaunch<CholK<T, NR, NC>>(a, dim3(a.batch), dim3(64 * WAVES), 0, s);
 }
 
 template <typename T>
@@ -581,7 +589,8 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
     // up to 64 columns, fewer waves per matrix and more matrices per CU: 8 waves, or 4 once
     // there are more matrices than CUs (2048 x 64^2: 0.28 against 0.36 ms with 16 waves;
     // 1024 x 40^2: 0.10 against 0.12 ms)
-    const bool many = a.batch > cu_count();
+    // a grouped call launches the matrices of all its members together
+    const bool many = a.batch > cu_count() || grouping();
     if (a.N <= 32) return many ? run_gj<T, 1, 8, 4, 8>(a, s) : run_gj<T, 1, 4, 8, 4>(a, s);
     if (a.N <= 64) return many ? run_gj<T, 1, 16, 4, 16>(a, s) : run_gj<T, 1, 8, 8, 8>(a, s);
     if (a.N <= 96) return many ? run_gj<T, 2, 12, 8, 6>(a, s) : run_gj<T, 2, 6>(a, s);

@@ -300,13 +300,14 @@ def hetero_roofline(instances, nw, steps_per_s):
             "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["mixed"]}
 
 
-def run_hetero(instances, nw, steps, seed, tuned_graph):
-    """All instances for `steps` steps (QR every 10, comb every 40).  tuned_graph=False: default
-    tiles, per-instance launches on one stream.  True: autotuned tiles, each instance on its own
-    stream, every 10-step block of all instances one HIP-graph launch."""
-    from afqmclab_amd.cp import Context, CPMethod, CPPropagator
+def run_hetero(instances, nw, steps, seed, mode, group_tiles=(0, 0, 0)):
+    """All instances for `steps` steps (QR every 10, comb every 40).  mode "eager": default tiles,
+    per-instance launches on one stream.  "graph": autotuned tiles, each instance on its own
+    stream, every 10-step block of all instances one HIP-graph launch.  "grouped": every kernel of
+    a phase launched once over all instances (afqmc_cp_group_*), every 10-step block one graph."""
+    from afqmclab_amd.cp import Context, CPGroup, CPMethod, CPPropagator
     origin = Context(0)
-    ctxs = [Context(0) for _ in instances] if tuned_graph else [origin] * len(instances)
+    ctxs = [Context(0) for _ in instances] if mode == "graph" else [origin] * len(instances)
     sets = []
     tiles = []
     for i, ((name, model, phi, nup), c) in enumerate(zip(instances, ctxs)):
@@ -320,14 +321,24 @@ def run_hetero(instances, nw, steps, seed, tuned_graph):
         prop.pop_control(ws)
         prop.onebody(ws, 1)
         sets.append((prop, ws))
-    if tuned_graph:
+    if mode == "graph":
         for prop, ws in sets:
             t, _ = prop.autotune(ws, reps=3)
             tiles.append([CPPropagator.TILES[k] for k in t])
+    grp = CPGroup(sets, tiles=group_tiles) if mode == "grouped" else None
+    if grp is not None:
+        tiles = [CPPropagator.TILES[k] for k in group_tiles]
     for c in set(ctxs):
         c.synchronize()
 
     def block(with_comb):
+        if grp is not None:
+            for _ in range(10):
+                grp.step()
+            grp.stabilize()
+            if with_comb:
+                grp.pop_control()
+            return
         for prop, ws in sets:
             for _ in range(10):
                 prop.step(ws)
@@ -336,16 +347,16 @@ def run_hetero(instances, nw, steps, seed, tuned_graph):
                 prop.pop_control(ws)
 
     graphs = {}
-    if tuned_graph:
+    if mode != "eager":
         for with_comb in (False, True):
-            origin.capture_begin(members=ctxs)
+            origin.capture_begin(members=ctxs if mode == "graph" else ())
             block(with_comb)
             graphs[with_comb] = origin.capture_end()
     nblocks = steps // 10
     t0 = time.perf_counter()
     for b in range(nblocks):
         with_comb = (b + 1) % 4 == 0
-        if tuned_graph:
+        if graphs:
             graphs[with_comb].launch()
         else:
             block(with_comb)
@@ -354,9 +365,11 @@ def run_hetero(instances, nw, steps, seed, tuned_graph):
         c.synchronize()
     elapsed = time.perf_counter() - t0
     logo = [prop.log_overlap(ws) for prop, ws in sets]
-    nodes = graphs[False].nodes() if tuned_graph else None
+    nodes = graphs[False].nodes() if graphs else None
     for g in graphs.values():
         g.close()
+    if grp is not None:
+        grp.close()
     for prop, ws in sets:
         ws.close()
         prop.close()
@@ -565,6 +578,8 @@ def main():
     ap.add_argument("--ph-steps", type=int, default=60, help="phaseless leg steps (0 = skip)")
     ap.add_argument("--ph-walkers", type=int, default=1024)
     ap.add_argument("--hetero-steps", type=int, default=200, help="heterogeneous batch leg steps (0 = skip)")
+    ap.add_argument("--hetero-tiles", type=int, nargs=3, default=[0, 0, 0], metavar=("PROP", "OVL", "GREEN"),
+                    help="tile shapes of the grouped heterogeneous leg (as afqmc_cp_set_tiles)")
     ap.add_argument("--extras", type=int, default=1, help="run the 4x4-vs-ED and 8x8 gradient legs (0 = skip)")
     ap.add_argument("--hs-steps", type=int, default=6, help="launches of the spin-mixing HS kernel in its bandwidth leg (0 = skip)")
     ap.add_argument("--bp-measurements", type=int, default=3,
@@ -702,11 +717,16 @@ def main():
     if args.hetero_steps > 0:
         inst = hetero_instances()
         log(f"[bench] heterogeneous batch: {len(inst)} Hubbard instances x 256 walkers, {args.hetero_steps} steps")
-        eager = run_hetero(inst, 256, args.hetero_steps, args.seed, False)
+        eager = run_hetero(inst, 256, args.hetero_steps, args.seed, "eager")
         log(f"[bench]   per-instance launches, default tiles: {eager['walker_steps_per_s']:.0f} walker-steps/s")
-        graph = run_hetero(inst, 256, args.hetero_steps, args.seed, True)
+        graph = run_hetero(inst, 256, args.hetero_steps, args.seed, "graph")
         log(f"[bench]   one graph per 10-step block, autotuned tiles: {graph['walker_steps_per_s']:.0f} walker-steps/s")
+        grouped = run_hetero(inst, 256, args.hetero_steps, args.seed, "grouped", tuple(args.hetero_tiles))
+        log(f"[bench]   grouped launches, one graph per 10-step block: {grouped['walker_steps_per_s']:.0f} "
+            f"walker-steps/s, {grouped['graph_nodes']} graph nodes per block")
         same = all(np.allclose(a, b, rtol=1e-12, atol=0) for a, b in zip(eager["logo"], graph["logo"]))
+        same_grouped = all(np.array_equal(a, b) for a, b in zip(eager["logo"], grouped["logo"]))
+        best = max((eager, graph, grouped), key=lambda r: r["walker_steps_per_s"])
         out["batched_hetero"] = {
             "workload": "24 Hubbard instances 6x6..12x12 (6 each), filling 0.75-1.0, U in [2,8], t'=-0.2 on odd "
                         "instances, 256 walkers each (6144), dt=0.01, mixed precision, QR every 10, comb every 40",
@@ -714,11 +734,16 @@ def main():
             "per_instance_launches": {"walker_steps_per_s": eager["walker_steps_per_s"], "seconds": eager["elapsed"]},
             "graph_autotuned": {"walker_steps_per_s": graph["walker_steps_per_s"], "seconds": graph["elapsed"],
                                 "graph_nodes_per_block": graph["graph_nodes"], "tiles": graph["tiles"]},
-            "speedup": graph["walker_steps_per_s"] / eager["walker_steps_per_s"],
+            "grouped_graph": {"walker_steps_per_s": grouped["walker_steps_per_s"], "seconds": grouped["elapsed"],
+                              "graph_nodes_per_block": grouped["graph_nodes"], "tiles": grouped["tiles"]},
+            "speedup_graph": graph["walker_steps_per_s"] / eager["walker_steps_per_s"],
+            "speedup_grouped": grouped["walker_steps_per_s"] / eager["walker_steps_per_s"],
             # algorithmic flops of all instances per step (spin-block form, QR every 10, comb every
-            # 40, no measurement) over the graph path's time per step, against the FP32 MFMA peak
-            "roofline": hetero_roofline(inst, 256, graph["walker_steps_per_s"] / (len(inst) * 256)),
+            # 40, no measurement) over the time per step, against the FP32 MFMA peak
+            "roofline": hetero_roofline(inst, 256, grouped["walker_steps_per_s"] / (len(inst) * 256)),
+            "roofline_graph": hetero_roofline(inst, 256, graph["walker_steps_per_s"] / (len(inst) * 256)),
             "same_log_overlaps": bool(same),
+            "grouped_identical_to_per_instance": bool(same_grouped),
         }
     if args.hs_steps > 0:
         out["hs_apply_2x2"] = run_hs_bandwidth(ctx, args.walkers, args.hs_steps)

@@ -137,6 +137,26 @@ int afqmc_cp_get_ET(afqmc_cp cp, double* ET);
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles);
 int afqmc_cp_autotune(afqmc_cp cp, afqmc_walkers w, int reps, int* tiles, double* ms);
 
+/* Groups of runs (no reference counterpart: the reference runs one model per process; this is for
+ * callers that propagate many Hubbard instances of uneven size at once).  A group holds n members,
+ * each a propagator with one of its walker sets, all on one context and in one precision (FP64 or
+ * mixed; not the split-BF16 propagation), no two sharing a propagator.  group_step, group_stabilize
+ * and group_pop_control do what afqmc_cp_step, afqmc_cp_stabilize and afqmc_cp_pop_control (without
+ * outputs) do on every member, with the same results, but launch each kernel of a phase once over
+ * all members (about ten launches per step instead of ten per member), which a graph captured from
+ * group calls replays.  The group's tile shapes (set_tiles, as afqmc_cp_set_tiles; 64x64 by default)
+ * stand in for the members' own during its calls.  Argument tables of the grouped launches are kept
+ * on the device until group_destroy (table_count: how many), so graphs captured from a group's calls
+ * must be destroyed before it.  After an error the members' walker sets are undefined. */
+typedef struct afqmc_cp_group_s* afqmc_cp_group;
+int afqmc_cp_group_create(int n, const afqmc_cp* cps, const afqmc_walkers* walkers, afqmc_cp_group* out);
+int afqmc_cp_group_destroy(afqmc_cp_group g);
+int afqmc_cp_group_set_tiles(afqmc_cp_group g, const int* tiles);
+int afqmc_cp_group_step(afqmc_cp_group g);
+int afqmc_cp_group_stabilize(afqmc_cp_group g);
+int afqmc_cp_group_pop_control(afqmc_cp_group g);
+int afqmc_cp_group_table_count(afqmc_cp_group g);
+
 /* SDSDOperation::returnLogOverlap (SDSDOperation.cpp:65-72) for every walker: logO nw complex */
 int afqmc_cp_log_overlap(afqmc_cp cp, afqmc_walkers w, double* logO);
 /* SDSDOperation::returnGreenDiagonal / returnGreenOffDiagonal (SDSDOperation.cpp:91-141):

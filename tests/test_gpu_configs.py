@@ -24,7 +24,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-from afqmclab_amd.cp import Context, CPMethod, CPPropagator  # noqa: E402
+from afqmclab_amd.cp import Context, CPGroup, CPMethod, CPPropagator  # noqa: E402
 from afqmclab_amd.model import HubbardModel, heterogeneous_hubbard_instances, uhf_trial  # noqa: E402
 from afqmclab_amd.phaseless import MoleculeModel, PhaselessMethod, PhaselessPropagator  # noqa: E402
 from oracle import comb as ocomb  # noqa: E402
@@ -344,3 +344,104 @@ def test_heterogeneous_batch_graph_replays_equal_eager_calls():
         wb, lwb, alb = b[4].download_some([0, 100, 255])
         assert np.array_equal(wa, wb) and np.array_equal(lwa, lwb) and np.array_equal(ala, alb)
     g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_grouped_step_matches_oracle(precision):
+    """One grouped step of the 24 instances (every kernel of a phase launched once over all of
+    them) against the oracle's step of one sampled walker per instance."""
+    nw = 256
+    tol = tol_of(precision)
+    c = Context(0)
+    sets = hetero_sets(precision, nw, [c] * 24)
+    rng = np.random.default_rng(29)
+    us = []
+    for name, model, phi, prop, ws in sets:
+        u = rng.uniform(0, 1, model.L * nw)
+        prop.set_uniforms(u)
+        us.append(u)
+    grp = CPGroup([(prop, ws) for *_, prop, ws in sets])
+    print(f"[c4 grouped] precision={precision}: one grouped step", flush=True)
+    grp.step()
+    c.synchronize()
+    for j, (name, model, phi, prop, ws) in enumerate(sets):
+        k = (53 * j + 11) % nw
+        wf, lw, al = ws.download_some([k])
+        om = HubbardSOC(model.L, model.N, model.K, U=model.U)
+        drv = OracleCP(OracleMethod(dt=0.01, decompType="densitySpin", walkerSizePerThread=1, ET=-1.0,
+                                    thermalSize=0, ETAdjustMaxSize=0), om, ListStream(us[j]), phiT=SD(wf=phi))
+        drv.walker = [SD(wf=phi.copy())]
+        drv.walkerIsAlive = [True]
+        drv.rng.counter = k * model.L
+        drv.step_walker(0)
+        assert al[0] == 1 and drv.walkerIsAlive[0], name
+        ref = drv.walker[0]
+        assert np.max(np.abs(wf[0] - ref.wf)) < tol * np.max(np.abs(ref.wf)), name
+        close_log(lw[0], ref.logw, tol)
+    grp.close()
+    for *_, prop, ws in sets:
+        ws.close()
+        prop.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_grouped_graph_equals_per_instance_calls(precision):
+    """Blocks of 10 grouped steps + grouped QR (+ grouped comb every other block), captured as
+    graphs and replayed with eager per-instance work between the replays, against the same
+    per-instance calls made eagerly on identical propagators: log-overlaps, walkers, weights and
+    alive flags identical.  The block without the comb stays within 400 graph nodes."""
+    nw = 256
+    c = Context(0)
+    gsets = hetero_sets(precision, nw, [c] * 24)
+    esets = hetero_sets(precision, nw, [ctx()] * 24)
+    for j, (a, b) in enumerate(zip(gsets, esets)):
+        a[3].set_seed(300 + j, 0)
+        b[3].set_seed(300 + j, 0)
+    grp = CPGroup([(prop, ws) for *_, prop, ws in gsets])
+
+    def eager_block(sets, comb):
+        for *_, prop, ws in sets:
+            for _ in range(10):
+                prop.step(ws)
+            prop.stabilize(ws)
+            if comb:
+                prop.pop_control(ws)
+
+    def group_block(comb):
+        for _ in range(10):
+            grp.step()
+        grp.stabilize()
+        if comb:
+            grp.pop_control()
+
+    # eager grouped calls first (they also create the argument tables), then graphs of them
+    group_block(True)
+    eager_block(esets, True)
+    graphs = {}
+    for comb in (False, True):
+        c.capture_begin()
+        group_block(comb)
+        graphs[comb] = c.capture_end()
+    nodes = graphs[False].nodes()
+    print(f"[c4 grouped] precision={precision}: {nodes} graph nodes per 10-step block, "
+          f"{grp.table_count()} argument tables", flush=True)
+    assert nodes <= 400
+    for rep in range(4):
+        comb = rep % 2 == 1
+        graphs[comb].launch()
+        c.synchronize()
+        eager_block(esets, comb)
+        for a, b in zip(gsets, esets):
+            la, lb = a[3].log_overlap(a[4]), b[3].log_overlap(b[4])
+            assert np.array_equal(la, lb), (rep, a[0])
+    for a, b in zip(gsets, esets):
+        wa, lwa, ala = a[4].download_some([0, 100, 255])
+        wb, lwb, alb = b[4].download_some([0, 100, 255])
+        assert np.array_equal(wa, wb) and np.array_equal(lwa, lwb) and np.array_equal(ala, alb), a[0]
+    for g in graphs.values():
+        g.close()
+    grp.close()
+    for s in (gsets, esets):
+        for *_, prop, ws in s:
+            ws.close()
+            prop.close()
