@@ -24,6 +24,33 @@ namespace {
 // waves), and more waves let one wave's prologue and epilogue overlap the others' MFMAs, which
 // matters because the K loop of a Green tile is only N/16 long.  The larger wave tiles already
 // sit at 256 registers.
+// One 4-, 8- or 16-byte element through a buffer resource: voff is the lane's byte offset, soff a
+// wave-uniform byte offset (the K advance)
+template <typename V>
+__device__ __forceinline__ V buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  static_assert(sizeof(V) == 4 || sizeof(V) == 8 || sizeof(V) == 16, "element size");
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  } else if constexpr (sizeof(V) == 8) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  } else {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
+}
+// A raw buffer resource over 2^31 - 1 bytes from base (dword 3 of the gfx9 descriptor: 0x00020000).
+// base must be wave-uniform; it is moved to scalar registers explicitly, because a uniform value
+// that the compiler computed in vector registers (the tile index's 64-bit division) would
+// otherwise be wrapped in a loop over its distinct lane values.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  const unsigned long long a = reinterpret_cast<unsigned long long>(base);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  void* u = reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(u, (short)0, 0x7fffffff, 0x00020000);
+}
+
 template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
 struct CgemmK {
   static constexpr int NT = WM * WN * 64;
@@ -124,6 +151,58 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
       s_base[e] = (long long)((n0 + b_n + e * BE) / p.scale_group) * p.scale_strideG + b_k;
   }
 
+  // Buffer-resource loads (p.fast_loads) for the whole K tiles: each element's byte offset from the
+  // workgroup's operand base (row m0 of A, column n0 of B) is fixed over the K loop and the K
+  // advance is a scalar offset, so a K tile's loads cost no vector address arithmetic.  Elements
+  // outside M or N read the base and are zeroed at the store; a tile wholly inside M and N stores
+  // unmasked.  A partial last K tile takes the masked loads.
+  constexpr int AESZ = OPA == OP_NR ? (int)sizeof(T) : (int)sizeof(ct);
+  // Only where it measured faster on MI355X: FP32 op-N tiles of up to 64 x 64 with a plain store
+  // (the propagation GEMM's 64 x 64 tile: 0.93 against 0.99 ms per launch at 16 x 16 sites, 112
+  // electrons per spin and 2048 walkers).  The offset registers cost the larger and the FP64 tiles
+  // occupancy or spills, and the Green and overlap kernels a wave per SIMD.
+  constexpr bool FAST_OK = std::is_same<T, float>::value && OPA == OP_N && EPI == EPI_STORE && BM * BN <= 64 * 64;
+  const bool fast = FAST_OK && p.fast_loads != 0;
+  const bool tile_in = m0 + BM <= p.M && n0 + BN <= p.N;
+  const long long a_wg = A_NMAJ ? (long long)m0 : (long long)m0 * p.lda;
+  const __amdgpu_buffer_rsrc_t rsa =
+      buf_rsrc(OPA == OP_NR ? (const void*)(Ar + a_wg) : (const void*)(A + a_wg));
+  const __amdgpu_buffer_rsrc_t rsb = buf_rsrc(Bm + (long long)n0 * p.ldb);
+  const __amdgpu_buffer_rsrc_t rss = buf_rsrc(SCALE ? (const void*)S : (const void*)Bm);
+  int a_vo[A_ELEMS], b_vo[B_ELEMS], s_vo[SCALE ? B_ELEMS : 1];
+  if (fast) {
+#pragma unroll
+    for (int e = 0; e < A_ELEMS; ++e) {
+      const int am = a_m + (A_NMAJ ? 0 : e * AE), ak = a_k + (A_NMAJ ? e * AE : 0);
+      const bool in = m0 + am < p.M;
+      ina[e] = in;
+      const long long o = A_NMAJ ? (long long)am + (long long)ak * p.lda : (long long)ak + (long long)am * p.lda;
+      a_vo[e] = in ? (int)(o * AESZ) : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      const int bn = b_n + e * BE;
+      const bool in = n0 + bn < p.N;
+      inb[e] = in;
+      b_vo[e] = in ? (int)(((long long)b_k + (long long)bn * p.ldb) * (long long)sizeof(ct)) : 0;
+      if (SCALE) s_vo[e] = in ? (int)(s_base[e] * (long long)sizeof(ct)) : 0;
+    }
+  }
+  auto load_fast = [&](int k0) {
+    const int sa = __builtin_amdgcn_readfirstlane((int)((long long)k0 * a_kstride * AESZ));
+    const int sb = __builtin_amdgcn_readfirstlane(k0 * (int)sizeof(ct));
+#pragma unroll
+    for (int e = 0; e < A_ELEMS; ++e) {
+      if (OPA == OP_NR) ra[e] = cmk<T>(buf_load<T>(rsa, a_vo[e], sa), (T)0);
+      else ra[e] = buf_load<ct>(rsa, a_vo[e], sa);
+    }
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      rb[e] = buf_load<ct>(rsb, b_vo[e], sb);
+      if (SCALE) rs[e] = buf_load<ct>(rss, s_vo[e], sb);
+    }
+  };
+
   auto load_tiles = [&](int k0) {
     if (!kpos) {
 #pragma unroll
@@ -151,14 +230,15 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
       if (SCALE) rs[e] = S[in ? s_base[e] + k0 : 0];
     }
   };
-  auto store_tiles = [&]() {
+  auto store_tiles_as = [&](auto masked) {
+    constexpr bool MASKED = decltype(masked)::value;
 #pragma unroll
     for (int e = 0; e < A_ELEMS; ++e) {
       int idx = tid + e * NT;
       int m, k;
       if (OPA == OP_N || OPA == OP_NR) { m = idx % BM; k = idx / BM; }
       else { k = idx % BK; m = idx / BK; }
-      const ct v = ina[e] ? ra[e] : cmk<T>(0, 0);
+      const ct v = (!MASKED || ina[e]) ? ra[e] : cmk<T>(0, 0);
       As[0][k][m] = v.re;
       As[1][k][m] = (OPA == OP_C) ? -v.im : v.im;
     }
@@ -167,22 +247,34 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
       int idx = tid + e * NT;
       int k = idx % BK, n = idx / BK;
       ct v = cmk<T>(0, 0);
-      if (inb[e]) v = SCALE ? cmul(rs[e], rb[e]) : rb[e];
+      if (!MASKED || inb[e]) v = SCALE ? cmul(rs[e], rb[e]) : rb[e];
       Bs[0][k][n] = v.re;
       Bs[1][k][n] = v.im;
     }
+  };
+  // the masks of the buffer path are fixed over the K loop: a tile inside M and N needs none
+  auto store_tiles = [&](bool loaded_fast) {
+    if (loaded_fast && tile_in) store_tiles_as(std::false_type{});
+    else store_tiles_as(std::true_type{});
+  };
+  const int nk_whole = p.K / BK;  // K tiles with no rows past K
+  auto load = [&](int kt) {
+    const bool f = fast && kt < nk_whole;
+    if (f) load_fast(kt * BK);
+    else load_tiles(kt * BK);
+    return f;
   };
 
   int nk = (p.K + BK - 1) / BK;
   // upper-triangular B: rows k > n of column n are zero, so K tiles past this tile's last
   // column add nothing
   if (p.b_upper) nk = min(nk, (min(n0 + BN, p.K) + BK - 1) / BK);
-  load_tiles(0);
-  store_tiles();
+  bool next_fast = load(0);
+  store_tiles(next_fast);
   __syncthreads();
   const int lr = lane & 15, lk = lane >> 4;
   for (int kt = 0; kt < nk; ++kt) {
-    if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+    if (kt + 1 < nk) next_fast = load(kt + 1);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T are[FM], aim[FM], asum[FM], bre[FN], bim[FN], bsum[FN];
@@ -221,7 +313,7 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
     }
     __syncthreads();
     if (kt + 1 < nk) {
-      store_tiles();
+      store_tiles(next_fast);
       __syncthreads();
     }
   }
@@ -589,13 +681,34 @@ __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(GemmArgs p) {
     }
 }
 
+// whether the kernel may load its whole K tiles through buffer resources: every byte offset from a
+// workgroup's operand base (plus the K advance) below 2^31
+template <typename T, int OPA, int BM, int BN, int BK>
+bool fast_loads_ok(const GemmArgs& a, bool scale) {
+  if (a.K < BK) return false;
+  const long long lim = (1ll << 31) - 1;
+  const long long ce = (long long)sizeof(cplx<T>);
+  const long long ae = OPA == OP_NR ? (long long)sizeof(T) : ce;
+  const bool nmaj = OPA == OP_N || OPA == OP_NR;
+  const long long a_span = nmaj ? (long long)BM + (long long)a.K * a.lda : (long long)a.K + (long long)BM * a.lda;
+  const long long b_span = (long long)a.K + (long long)BN * a.ldb;
+  if (a_span * ae > lim || b_span * ce > lim) return false;
+  if (scale) {
+    const long long s_span = (long long)((a.N + a.scale_group - 1) / a.scale_group) * a.scale_strideG + a.K;
+    if (s_span * ce > lim) return false;
+  }
+  return true;
+}
+
 template <typename T, int OPA, int BM, int BN, int BK, int WM, int WN, int EPI, bool SCALE>
-int run(const GemmArgs& a, hipStream_t s) {
-  dim3 grid((a.M + BM - 1) / BM, (a.N + BN - 1) / BN, a.batch);
+int run(const GemmArgs& a0, hipStream_t s) {
+  dim3 grid((a0.M + BM - 1) / BM, (a0.N + BN - 1) / BN, a0.batch);
   if (grid.y > 65535 || grid.z > 65535) {
     afqmc_set_error("cgemm grid too large (%u, %u, %u)", grid.x, grid.y, grid.z);
     return -1;
   }
+  GemmArgs a = a0;
+  a.fast_loads = fast_loads_ok<T, OPA, BM, BN, BK>(a0, SCALE) ? 1 : 0;
   return glaunch<CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>>(a, grid, dim3(WM * WN * 64), 0, s);
 }
 

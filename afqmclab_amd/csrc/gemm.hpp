@@ -44,6 +44,9 @@ struct GemmArgs {
   // each real product formed as hi*hi + hi*lo + lo*hi with BF16 MFMAs accumulating in FP32
   // (fixed 128x64 tile); about 1e-6 of sum |a b| instead of FP32's 1e-7
   int split_bf16;
+  // set by the launcher, not the caller: every operand's byte offsets from a workgroup's base fit
+  // in 31 bits, so the kernel loads its whole K tiles through buffer resources
+  int fast_loads;
 };
 
 // Output tile shapes of the MFMA GEMM: the autotuner picks one per call site.

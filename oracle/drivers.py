@@ -588,7 +588,7 @@ class AfqmcPhaseless:
                 for k in range(m.measureSkipStep):
                     self.projectOneStep(idx)
             mm = self.mixedMeasure
-            self.energies.append((mm.HNum, mm.den))
+            self.energies.append((mm.HNum, mm.den, mm.TNum))
             mm.reSet()
 
     def run(self):

@@ -8,7 +8,9 @@ on sampled walkers:
   (the per-walker blocks of all 1024 walkers exceed the 4 GiB chunk buffer), with the measured
   walkers placed on both sides of chunk boundaries;
 * 24 heterogeneous lattices (6 x 6 to 12 x 12) x 256 walkers: one graph step of all instances
-  against the oracle, and graph replays of 10 steps + QR + comb against eager calls.
+  against the oracle, and graph replays of 10 steps + QR + comb against eager calls; the same with
+  grouped launches (one launch per kernel and phase over all instances) against the oracle and,
+  bit for bit, against per-instance calls.
 
 A walker's uniforms start after those of the earlier walkers that pass the overlap-phase check
 (killed walkers draw none, afqmcConstraintPathProjection.cpp:47), so each sampled walker is

@@ -308,7 +308,7 @@ def test_driver_trajectory_matches_oracle(schedule):
     assert dev.prop.counter() == ref.rng.counter
     assert len(dev.energies) == len(ref.energies)
     for a, b in zip(dev.energies, ref.energies):
-        for x, y in zip(a[:2], b[:2]):
+        for x, y in zip(a[:3], b[:3]):
             assert abs(x - y) < 1e-9 * abs(y), (x, y)
     if schedule == "adjust_then_growth":
         assert np.allclose(dev.model.bg, ref.model.choleskyBg, rtol=1e-9, atol=1e-12)

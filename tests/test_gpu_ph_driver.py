@@ -40,9 +40,11 @@ def read_model_bg(d):
 
 
 def write_param(d, m):
+    """afqmc_param as PhaselessMethod::write lays it out (afqmcPhaselessMethod.cpp:55-90): the key and a
+    space left-aligned in 36 columns, the value in 26."""
     with open(os.path.join(d, "afqmc_param"), "w") as f:
         for k, v in m.items():
-            f.write(f"{k:36s}{v}\n")
+            f.write(f"{k + ' ':<36s}{str(v):<26s}\n")
 
 
 def write_sd2s(path, up, dn, logw=0j):
@@ -96,7 +98,8 @@ def test_ph_run_matches_oracle(tmp_path, schedule):
         parts = line.split()
         if parts and parts[0] == "ET":
             et = float(parts[1])
-    assert et is not None and abs(et - ref.method.ET) < 1e-9 * abs(ref.method.ET)
+    # written as PhaselessMethod::write writes it: the stream's default six significant digits
+    assert et is not None and et == float(f"{ref.method.ET:g}")
     # walkers written for the alive ones, in the SD2is format
     files = os.listdir(os.path.join(d, "walkers"))
     assert len(files) == sum(ref.walkerIsAlive)
@@ -122,7 +125,9 @@ def test_ph_run_trial_from_model_and_restart(tmp_path):
     ref = OraclePh(OracleMethod(**m), om, UniformStream(5))
     with open(os.path.join(d, "phiT.dat")) as f:
         vals = f.read().split()
-    up = np.array([float(vals[6 + 2 * k]) + 1j * float(vals[7 + 2 * k]) for k in range(30)]).reshape(3, 10).T
+    # SD2s::write (SD2s.cpp:114-123): logw (2 numbers), then rank 2, L, Nup and the up elements
+    assert (int(vals[2]), int(vals[3]), int(vals[4])) == (2, 10, 3)
+    up = np.array([float(vals[5 + 2 * k]) + 1j * float(vals[6 + 2 * k]) for k in range(30)]).reshape(3, 10).T
     for j in range(3):
         c = np.vdot(ref.phiT.wfUp[:, j], up[:, j])
         assert abs(abs(c) - 1) < 1e-10
