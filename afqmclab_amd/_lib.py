@@ -144,6 +144,7 @@ SIGNATURES = {
     "afqmc_test_zgemm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_longlong,
                                    _dp, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "afqmc_test_gj_inverse": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip]),
+    "afqmc_test_gj_inverse_layout": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip]),
     "afqmc_test_copy": (C.c_int, [_vp, C.c_longlong, C.c_int, _dp]),
     "afqmc_test_chol_inverse": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _ip]),
 }

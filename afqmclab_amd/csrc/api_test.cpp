@@ -75,7 +75,8 @@ int zgemm_t(afqmc_ctx ctx, int opa, int M, int N, int K, int batch, const double
 }
 
 template <typename T>
-int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ainv, double* logdet, int* info) {
+int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ainv, double* logdet, int* info,
+         int layout = 0) {
   hipStream_t s = ctx->stream;
   const size_t n = (size_t)N * N * batch;
   auto h = narrow<T>(A, n);
@@ -91,7 +92,7 @@ int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ain
   memset(&a, 0, sizeof(a));
   a.N = N; a.batch = batch; a.in = d.p; a.ld_in = N; a.stride_in = (long long)N * N;
   a.out = d.p; a.ld_out = N; a.stride_out = a.stride_in; a.logdet = ld.p; a.info = inf.p;
-  a.write_inverse = 1; a.nsub = N; a.alive_div = 1;
+  a.write_inverse = 1; a.nsub = N; a.alive_div = 1; a.layout = layout;
   rc = kind == 0 ? launch_gj_inverse<T>(a, s) : launch_chol_inverse<T>(a, s);
   if (rc) return rc;
   std::vector<cplx<T>> o(n);
@@ -127,6 +128,14 @@ int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const 
   ScopedDevice sd(ctx);
   return precision ? gj_t<float>(ctx, 0, N, batch, A, Ainv, logdet, info)
                    : gj_t<double>(ctx, 0, N, batch, A, Ainv, logdet, info);
+}
+
+int afqmc_test_gj_inverse_layout(afqmc_ctx ctx, int precision, int N, int batch, int layout, const double* A,
+                                 double* Ainv, double* logdet, int* info) {
+  REQUIRE(ctx && A && Ainv && logdet, "null argument");
+  ScopedDevice sd(ctx);
+  return precision ? gj_t<float>(ctx, 0, N, batch, A, Ainv, logdet, info, layout)
+                   : gj_t<double>(ctx, 0, N, batch, A, Ainv, logdet, info, layout);
 }
 
 int afqmc_test_copy(afqmc_ctx ctx, long long bytes, int reps, double* ms) {

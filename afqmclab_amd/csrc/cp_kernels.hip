@@ -344,28 +344,55 @@ __device__ __forceinline__ void HsApplyK<T, S>::run(const BlockIdx3& bk, const H
   __syncthreads();
   const int j0 = bk.x * cpb, j1 = min(N, j0 + cpb);
   const long long base = (long long)w * 2 * L * N;
-  auto item = [&](int j, int v) {
-    const long long o = base + (long long)j * 2 * L + (long long)v * S;
-    const vt a = *reinterpret_cast<const vt*>(in + o);
-    const vt b = *reinterpret_cast<const vt*>(in + o + L);
-    const vt d00 = sd[v], d01 = sd[nv + v], d10 = sd[2 * nv + v], d11 = sd[3 * nv + v];
-    vt up, dn;
+  auto apply = [&](const vt& a, const vt& b, const vt& d00, const vt& d01, const vt& d10, const vt& d11, vt& up,
+                   vt& dn) {
 #pragma unroll
     for (int t = 0; t < S; ++t) {
       up.z[t] = cadd(cmul(d00.z[t], a.z[t]), cmul(d01.z[t], b.z[t]));
       dn.z[t] = cadd(cmul(d10.z[t], a.z[t]), cmul(d11.z[t], b.z[t]));
     }
-    *reinterpret_cast<vt*>(out + o) = up;
-    *reinterpret_cast<vt*>(out + o + L) = dn;
   };
   if (nv <= (int)blockDim.x) {
-    // several columns per pass: thread -> (column offset, site vector), fixed over the loop
+    // thread -> (column offset, site vector), fixed over the loop, so its site matrices are read
+    // from LDS once.  Four columns per iteration with every load issued before the first store:
+    // eight 16-byte loads in flight per thread (one column at a time left the CU short of
+    // outstanding loads: 0.59-0.62 of the HBM peak).
+    constexpr int U = 4;
     const int cpp = blockDim.x / nv, jj = tid / nv, v = tid - jj * nv;
-    if (jj < cpp)
-      for (int j = j0 + jj; j < j1; j += cpp) item(j, v);
+    if (jj < cpp) {
+      const vt d00 = sd[v], d01 = sd[nv + v], d10 = sd[2 * nv + v], d11 = sd[3 * nv + v];
+      for (int j = j0 + jj; j < j1; j += U * cpp) {
+        vt a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (j + u * cpp < j1) {
+            const long long o = base + (long long)(j + u * cpp) * 2 * L + (long long)v * S;
+            a[u] = *reinterpret_cast<const vt*>(in + o);
+            b[u] = *reinterpret_cast<const vt*>(in + o + L);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (j + u * cpp < j1) {
+            const long long o = base + (long long)(j + u * cpp) * 2 * L + (long long)v * S;
+            vt up, dn;
+            apply(a[u], b[u], d00, d01, d10, d11, up, dn);
+            *reinterpret_cast<vt*>(out + o) = up;
+            *reinterpret_cast<vt*>(out + o + L) = dn;
+          }
+        }
+      }
+    }
   } else {
     for (int j = j0; j < j1; ++j)
-      for (int v = tid; v < nv; v += blockDim.x) item(j, v);
+      for (int v = tid; v < nv; v += blockDim.x) {
+        const long long o = base + (long long)j * 2 * L + (long long)v * S;
+        vt up, dn;
+        apply(*reinterpret_cast<const vt*>(in + o), *reinterpret_cast<const vt*>(in + o + L), sd[v], sd[nv + v],
+              sd[2 * nv + v], sd[3 * nv + v], up, dn);
+        *reinterpret_cast<vt*>(out + o) = up;
+        *reinterpret_cast<vt*>(out + o + L) = dn;
+      }
   }
 }
 
@@ -1061,9 +1088,11 @@ int launch_hs_apply_2x2(int nw, int L, int N, const void* in, void* out, const v
   const bool pair = std::is_same<T, float>::value && L % 2 == 0;
   const int S = pair ? 2 : 1;
   const size_t lds = (size_t)4 * L * sizeof(cplx<T>);
-  // about 64 site vectors per thread per workgroup
+  // about 64 site vectors per thread per workgroup, in column chunks of equal width
   const int nv = L / S;
-  const int cpb = std::max(1, std::min(N, (64 * 256) / std::max(nv, 1)));
+  const int target = std::max(1, std::min(N, (64 * 256) / std::max(nv, 1)));
+  const int chunks = (N + target - 1) / target;
+  const int cpb = (N + chunks - 1) / chunks;
   dim3 grid((unsigned)((N + cpb - 1) / cpb), (unsigned)nw);
   HsApplyArgs<T> g;
   memset(&g, 0, sizeof(g));

@@ -22,6 +22,7 @@
 
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -45,6 +46,14 @@ struct GroupEntry {
   int gx, gy, gz, pad;
 };
 
+// F::WPE is the minimum number of waves per SIMD the register allocation must allow.  A functor
+// that also defines WPE_MAX bounds it from above (waves_per_eu(WPE, WPE_MAX)), which lets the
+// compiler use the registers of fewer waves.
+template <class F, class = void>
+struct has_wpe_max : std::false_type {};
+template <class F>
+struct has_wpe_max<F, std::void_t<decltype(F::WPE_MAX)>> : std::true_type {};
+
 template <class F, class Args>
 __global__ __launch_bounds__(F::NT) __attribute__((amdgpu_waves_per_eu(F::WPE))) void plain_kernel(Args a) {
   const BlockIdx3 b{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
@@ -66,6 +75,31 @@ __global__ __launch_bounds__(F::NT) __attribute__((amdgpu_waves_per_eu(F::WPE)))
   const int gx = tab[lo].gx, gy = tab[lo].gy, gz = tab[lo].gz;
   const int local = g - gi.start[lo];
   if (local >= gx * gy * gz) return;  // padding up to the next multiple of 8
+  const BlockIdx3 b{local % gx, (local / gx) % gy, local / (gx * gy), gx, gy, gz};
+  F::run(b, a);
+}
+
+template <class F, class Args>
+__global__ __launch_bounds__(F::NT) __attribute__((amdgpu_waves_per_eu(F::WPE, F::WPE_MAX))) void plain_kernel_b(
+    Args a) {
+  const BlockIdx3 b{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+  F::run(b, a);
+}
+
+template <class F, class Args>
+__global__ __launch_bounds__(F::NT) __attribute__((amdgpu_waves_per_eu(F::WPE, F::WPE_MAX))) void group_kernel_b(
+    const GroupEntry<Args>* __restrict__ tab, GroupIndex gi) {
+  const int g = (int)blockIdx.x;
+  int lo = 0, hi = gi.n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (gi.start[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const Args a = tab[lo].a;
+  const int gx = tab[lo].gx, gy = tab[lo].gy, gz = tab[lo].gz;
+  const int local = g - gi.start[lo];
+  if (local >= gx * gy * gz) return;
   const BlockIdx3 b{local % gx, (local / gx) % gy, local / (gx * gy), gx, gy, gz};
   F::run(b, a);
 }
@@ -160,8 +194,12 @@ int group_flush_kernel(const GroupRecord* const* r, int n, hipStream_t s, GroupT
     if (start == 0) continue;
     const void* d = t->get(tab.data(), sizeof(GroupEntry<Args>) * cn);
     REQUIRE(d, "grouped launch: no device memory for its argument table");
-    hipLaunchKernelGGL((group_kernel<F, Args>), dim3((unsigned)start), r[c0]->block, lds, s,
-                       reinterpret_cast<const GroupEntry<Args>*>(d), gi);
+    if constexpr (has_wpe_max<F>::value)
+      hipLaunchKernelGGL((group_kernel_b<F, Args>), dim3((unsigned)start), r[c0]->block, lds, s,
+                         reinterpret_cast<const GroupEntry<Args>*>(d), gi);
+    else
+      hipLaunchKernelGGL((group_kernel<F, Args>), dim3((unsigned)start), r[c0]->block, lds, s,
+                         reinterpret_cast<const GroupEntry<Args>*>(d), gi);
     hipError_t e = hipGetLastError();
     REQUIRE(e == hipSuccess, "grouped launch -> %s", hipGetErrorString(e));
   }
@@ -174,7 +212,8 @@ int glaunch(const Args& a, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
   if ((long long)grid.x * grid.y * grid.z == 0) return 0;
   if (g_group) {
     GroupRecord q;
-    q.key = reinterpret_cast<const void*>(&plain_kernel<F, Args>);
+    if constexpr (has_wpe_max<F>::value) q.key = reinterpret_cast<const void*>(&plain_kernel_b<F, Args>);
+    else q.key = reinterpret_cast<const void*>(&plain_kernel<F, Args>);
     q.flush = &group_flush_kernel<F, Args>;
     q.args.resize(sizeof(Args));
     memcpy(q.args.data(), &a, sizeof(Args));
@@ -187,7 +226,8 @@ int glaunch(const Args& a, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
     g_group->recs.push_back(std::move(q));
     return 0;
   }
-  hipLaunchKernelGGL((plain_kernel<F, Args>), grid, block, lds, s, a);
+  if constexpr (has_wpe_max<F>::value) hipLaunchKernelGGL((plain_kernel_b<F, Args>), grid, block, lds, s, a);
+  else hipLaunchKernelGGL((plain_kernel<F, Args>), grid, block, lds, s, a);
   hipError_t e = hipGetLastError();
   REQUIRE(e == hipSuccess, "launch -> %s", hipGetErrorString(e));
   return 0;

@@ -350,6 +350,239 @@ __device__ __forceinline__ void GjK<T, NR, NC, WV, NL>::run(const BlockIdx3& bk,
   }
 }
 
+// Gauss-Jordan, 2-D lane layout: lane = lc LR + lr of wave w holds rows lr + LR r (r < NR) of the
+// columns w + WV (lc + LC c) (c < NC, LC = 64 / LR).  A column lives in the LR lanes of one column
+// group of one wave, so its pivot search is still one wave reduction (the other lanes offer key
+// 0).  Both operands of the rank-1 update come from LDS: the multipliers of a lane's rows, which the
+// column's owner publishes, and the pivot row's values in the lane's columns, which the lanes
+// holding the pivot row write and their wave reads back.  Per step and lane that is NR NC complex
+// multiply-adds against NR + NC LDS reads and no readlanes (the 1-D layout, lanes = rows, reads NC
+// pivot-row values by four readlanes each).  Pivots, the update order of every element and the
+// deferred row scaling are those of the 1-D kernel, so the results agree with it bit for bit.
+template <typename T, int WV, int LR, int NR, int NC, int WPEV>
+struct Gj2K {
+  static constexpr int NT = 64 * WV, WPE = WPEV, WPE_MAX = WPEV;
+  __device__ static void run(const BlockIdx3& bk, const SmallDenseArgs& p);
+};
+template <typename T, int WV, int LR, int NR, int NC, int WPEV>
+__device__ __forceinline__ void Gj2K<T, WV, LR, NR, NC, WPEV>::run(const BlockIdx3& bk, const SmallDenseArgs& p) {
+  constexpr int LC = 64 / LR;
+  constexpr int ROWS = LR * NR;
+  constexpr int COLS = WV * LC * NC;
+  constexpr int PER = WV * LC;  // columns per register column index
+  static_assert(LR * LC == 64 && WV >= 2 && ROWS <= 256, "2-D layout");
+  const int bidx = bk.x;
+  if (p.alive && !p.alive[bidx / p.alive_div]) return;
+  const int N = p.N;
+  const int lane = threadIdx.x & 63;
+  const int lr = lane % LR, lc = lane / LR;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  typedef cplx<T> ct;
+  const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
+  ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
+
+  __shared__ zd colbuf[2][ROWS];
+  __shared__ zd prow[WV][LC][NR][NC];
+  __shared__ int pivrow[2];
+  __shared__ zd pivs[COLS];
+  __shared__ zd rowscale[ROWS];
+  __shared__ int perm[COLS];
+  __shared__ int qinv[ROWS];
+  __shared__ int singular, parity;
+
+  zd x[NR][NC];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int i = lr + LR * r, j = wv + WV * (lc + LC * c);
+      x[r][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+    }
+  unsigned used = 0u;  // bit r: row lr + LR r has been a pivot row
+  if (threadIdx.x == 0) { singular = 0; parity = 0; }
+
+  // The owner of column `col` (register column C1, column group lcg of this wave, updated through
+  // step col-1) picks its pivot among the unused rows and publishes the multipliers and the pivot.
+  auto publish = [&](auto c1c, int lcg, int col, int buf) __attribute__((always_inline)) {
+    constexpr int C1 = decltype(c1c)::value;
+    unsigned long long key = 0ull;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int i = lr + LR * r;
+      if (lc == lcg && i < N && !((used >> r) & 1u)) {
+        const unsigned long long kk = pivot_key(x[r][C1], i);
+        key = kk > key ? kk : key;
+      }
+    }
+    key = wave_max_u64(key);
+    const int bi = 255 - (int)(key & 0xFFull);
+    const int pr = (bi >= 0 && bi < N) ? bi : 0;
+    const int rr = pr / LR, pl = lcg * LR + pr % LR;
+    // every row register read by its own readlane and the result chosen among those values: a
+    // choice among the registers themselves (x[rr]) is compiled into a read through a run-time
+    // pointer, which moves x out of registers
+    zd piv = readlane_z(x[0][C1], pl);
+    static_for<NR>([&](auto rc) __attribute__((always_inline)) {
+      constexpr int R = decltype(rc)::value;
+      if constexpr (R > 0) {
+        const zd v = readlane_z(x[R][C1], pl);
+        piv.re = (rr == R) ? v.re : piv.re;  // componentwise: a select of the struct copies it through memory
+        piv.im = (rr == R) ? v.im : piv.im;
+      }
+    });
+    const bool nz = abs1(piv) > 0.0;
+    const double id = nz ? recip(piv.re * piv.re + piv.im * piv.im) : 0.0;
+    const zd inv = cmk<double>(piv.re * id, -piv.im * id);
+    if (lc == lcg) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int i = lr + LR * r;
+        colbuf[buf][i] = (i == pr) ? cmk<double>(0, 0) : cmul(x[r][C1], inv);
+      }
+    }
+    if (lane == 0) {
+      pivrow[buf] = pr;
+      pivs[col] = piv;
+      perm[col] = pr;
+      rowscale[pr] = cmk<double>(-inv.re, -inv.im);
+      if (!nz) singular = 1;
+    }
+  };
+
+  if (wv == 0) publish(std::integral_constant<int, 0>{}, 0, 0, 0);
+  __syncthreads();
+
+  // Step k = CB PER + t: column k is register column CB of column group t / WV of wave t % WV;
+  // column k+1 is register column CB of group (t+1) / WV of wave (t+1) % WV, or register column
+  // CB+1 of group 0 of wave 0 when t is the block's last.
+  auto step = [&](auto cbc, int t) __attribute__((always_inline)) {
+    constexpr int CB = decltype(cbc)::value;
+    const int k = CB * PER + t;
+    const int buf = k & 1, nbuf = buf ^ 1;
+    const int pr = __builtin_amdgcn_readfirstlane(pivrow[buf]);
+    const int rr = pr / LR, plr = pr % LR;
+    zd m[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) m[r] = colbuf[buf][lr + LR * r];
+    // the pivot row's values in this wave's columns, one lane per column group
+    // (all NR rows of the pivot row's lanes go to LDS and the readers pick row rr there, for the
+    // reason given in publish)
+    if (lr == plr) {
+#pragma unroll
+      for (int R = 0; R < NR; ++R)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) prow[wv][lc][R][c] = x[R][c];
+      used |= 1u << rr;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    zd rv[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) rv[c] = prow[wv][lc][rr][c];
+    const int kw = t % WV, kl = t / WV;  // owner of column k
+    const bool next_in_cb = t + 1 < PER;
+    const int tn = next_in_cb ? t + 1 : 0;
+    const bool next_here = (k + 1 < N) && (wv == tn % WV);
+    // look-ahead: the register column holding column k+1 first, then its pivot and multipliers
+    if (next_here) {
+      if (next_in_cb) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) x[r][CB] = cfms(x[r][CB], m[r], rv[CB]);
+        publish(std::integral_constant<int, CB>{}, tn / WV, k + 1, nbuf);
+      } else {
+        if constexpr (CB + 1 < NC) {
+#pragma unroll
+          for (int r = 0; r < NR; ++r) x[r][CB + 1] = cfms(x[r][CB + 1], m[r], rv[CB + 1]);
+          publish(std::integral_constant<int, CB + 1>{}, 0, k + 1, nbuf);
+        }
+      }
+    }
+    const bool owner = (wv == kw);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const bool done = next_here && ((next_in_cb && c == CB) || (!next_in_cb && c == CB + 1));
+      if (done) continue;
+      // column k itself (group kl of the owner's register column CB) becomes the multipliers
+      const bool keep = owner && c == CB && lc == kl;
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        if (!keep) x[r][c] = cfms(x[r][c], m[r], rv[c]);
+    }
+    if (owner && lc == kl) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) x[r][CB] = (lr + LR * r == pr) ? cmk<double>(-1.0, 0.0) : m[r];
+    }
+    __syncthreads();
+  };
+  auto block = [&](auto cbc) __attribute__((always_inline)) {
+    constexpr int CB = decltype(cbc)::value;
+    for (int t = 0; t < PER && CB * PER + t < N; ++t) step(cbc, t);
+  };
+  static_for<NC>(block);
+
+  for (int i = threadIdx.x; i < N; i += 64 * WV) qinv[perm[i]] = i;
+  // sign of the row permutation: parity of its inversions, counted in parallel
+  {
+    int odd = 0;
+    for (int e = threadIdx.x; e < N * N; e += 64 * WV) {
+      const int i = e / N, j = e - i * N;
+      odd ^= (i < j && perm[i] > perm[j]) ? 1 : 0;
+    }
+    if (odd) atomicXor(&parity, 1);
+  }
+  // log|det| and the phase of det from the pivots: wave 0, fixed-order trees (the 1-D kernel's)
+  double lognorm = 0.0;
+  zd phase = cmk<double>(1.0, 0.0);
+  if (wv == 0) {
+#pragma unroll
+    for (int q = 0; q < (NMAX + 63) / 64; ++q) {
+      const int k = lane + 64 * q;
+      if (k < N) {
+        const zd pv = pivs[k];
+        const double ap = cabs(pv);
+        const double ia = ap > 0.0 ? 1.0 / ap : 0.0;
+        lognorm += log(ap);
+        phase = cmul(phase, cmk<double>(pv.re * ia, pv.im * ia));
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lognorm += __shfl_xor(lognorm, o, 64);
+      const zd q = cmk<double>(__shfl_xor(phase.re, o, 64), __shfl_xor(phase.im, o, 64));
+      phase = cmul(phase, q);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const zd ph = parity ? cmk<double>(-phase.re, -phase.im) : phase;
+    zd ld;
+    if (singular) {
+      ld = cmk<double>(-1e300, 0.0);
+    } else {
+      ld = cmk<double>(lognorm, atan2(ph.im, ph.re));
+      ld.re += log(hypot(ph.re, ph.im));
+    }
+    p.logdet[bidx] = ld;
+    if (p.info) p.info[bidx] = singular;
+  }
+  if (!p.write_inverse) return;
+  // inverse: O^{-1}[qinv[i]][perm[j]] = x[i][j] / (-pivot of row i)
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int i = lr + LR * r;
+    if (i >= N) continue;
+    const int row = qinv[i];
+    const zd sc = rowscale[i];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int j = wv + WV * (lc + LC * c);
+      if (j >= N) continue;
+      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(x[r][c], sc));
+    }
+  }
+}
+
 // Cholesky and R^{-1}, row layout: y[c][r] = element (wave + 16c, lane + 64r).
 template <typename T, int NR, int NC>
 struct CholK {
@@ -575,6 +808,10 @@ template <typename T, int NR, int NC, int WV = WAVES, int NL = NC>
 int run_gj(const SmallDenseArgs& a, hipStream_t s) {
   return glaunch<GjK<T, NR, NC, WV, NL>>(a, dim3(a.batch), dim3(64 * WV), 0, s);
 }
+template <typename T, int WV, int LR, int NR, int NC, int WPE>
+int run_gj2(const SmallDenseArgs& a, hipStream_t s) {
+  return glaunch<Gj2K<T, WV, LR, NR, NC, WPE>>(a, dim3(a.batch), dim3(64 * WV), 0, s);
+}
 template <typename T, int NR, int NC>
 int run_chol(const SmallDenseArgs& a, hipStream_t s) {
   return glaunch<CholK<T, NR, NC>>(a, dim3(a.batch), dim3(64 * WAVES), 0, s);
@@ -586,6 +823,19 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
   REQUIRE(a.alive_div >= 1, "alive_div must be >= 1");
   if (a.batch <= 0) return 0;
   if (kind == 0) {
+    // The 2-D layout by default only up to 32 columns, where it measured faster on MI355X (3072
+    // inversions of 24 x 24: 0.070 against 0.101 ms).  From 33 columns on it is slower (40: 0.40
+    // against 0.26 ms; 112: 4.55 against 2.38 ms for 4096): its registers allow two waves per SIMD,
+    // and the LDS round trips of the multipliers and the pivot row sit on every step's chain.
+    const int layout = a.layout ? a.layout : (a.N <= 32 ? 2 : 1);
+    REQUIRE(layout == 1 || layout == 2, "Gauss-Jordan layout %d unknown", layout);
+    if (layout == 2 && a.N <= 128) {
+      if (a.N <= 32) return run_gj2<T, 2, 32, 1, 8, 4>(a, s);
+      if (a.N <= 64) return run_gj2<T, 4, 32, 2, 8, 2>(a, s);
+      if (a.N <= 96) return run_gj2<T, 8, 32, 3, 6, 2>(a, s);
+      if (a.N <= 112) return run_gj2<T, 8, 32, 4, 7, 2>(a, s);
+      return run_gj2<T, 8, 32, 4, 8, 2>(a, s);
+    }
     // up to 64 columns, fewer waves per matrix and more matrices per CU: 8 waves, or 4 once
     // there are more matrices than CUs (2048 x 64^2: 0.28 against 0.36 ms with 16 waves;
     // 1024 x 40^2: 0.10 against 0.12 ms)

@@ -382,6 +382,10 @@ int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K,
 /* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting) */
 int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,
                           double* logdet, int* info);
+/* the same with the kernel's lane layout chosen: 1 lanes = rows (readlanes for the pivot row), 2 a 2-D
+ * lane layout with both update operands from LDS (N <= 128); both give identical results */
+int afqmc_test_gj_inverse_layout(afqmc_ctx ctx, int precision, int N, int batch, int layout, const double* A,
+                                 double* Ainv, double* logdet, int* info);
 /* device-to-device streaming copy of `bytes` (a multiple of 16), reps times: the reference kernel
  * for scaling HBM traffic counters; ms = average kernel time */
 int afqmc_test_copy(afqmc_ctx ctx, long long bytes, int reps, double* ms);

@@ -67,6 +67,19 @@ def gj_inverse(A, precision=0):
     return np.transpose(out, (0, 2, 1)), ld, info
 
 
+def gj_inverse_layout(A, layout, precision=0):
+    """gj_inverse with the kernel's lane layout chosen (1: lanes = rows, 2: 2-D lane layout)."""
+    A = np.asarray(A, dtype=np.complex128)
+    batch, N, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    out = np.zeros_like(buf)
+    ld = np.zeros(batch, dtype=np.complex128)
+    info = np.zeros(batch, dtype=np.int32)
+    check(lib().afqmc_test_gj_inverse_layout(ctx().h, precision, N, batch, layout, dptr(buf), dptr(out), dptr(ld),
+                                             iptr(info)))
+    return np.transpose(out, (0, 2, 1)), ld, info
+
+
 def chol_inverse(S):
     S = np.asarray(S, dtype=np.complex128)
     batch, N, _ = S.shape

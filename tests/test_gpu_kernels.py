@@ -7,7 +7,7 @@ pytestmark = pytest.mark.gpu
 
 from oracle import comb as ocomb  # noqa: E402
 from oracle import linalg  # noqa: E402
-from tests.gpu_util import chol_inverse, crand, ctx, gj_inverse, zgemm  # noqa: E402
+from tests.gpu_util import chol_inverse, crand, ctx, gj_inverse, gj_inverse_layout, zgemm  # noqa: E402
 
 TOL = {0: 1e-13, 1: 2e-5}
 
@@ -181,6 +181,36 @@ def test_gj_inverse_large_batch(N, precision):
     if precision:
         assert np.array_equal(few[0], inv[:5])
     assert np.max(np.abs(few[1] - ld[:5])) < 1e-12 * N
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("N", [5, 17, 32, 33, 40, 63, 64, 72, 96, 100, 112, 128])
+def test_gj_two_d_layout_matches_one_d(N, precision):
+    """The 2-D lane layout (both operands of the rank-1 update from LDS) against the 1-D layout
+    (lanes = rows, the pivot row by readlane) on the same matrices, with a singular and a
+    permutation matrix among them: the same breakdown flags, log-determinants to 1e-13, inverses
+    to 1e-12 (FP64) or to FP32 rounding (FP32 storage), and inv A = I for the 2-D layout."""
+    rng = np.random.default_rng(50 + N + precision)
+    batch = 300
+    A = crand(rng, batch, N, N) + 0.5 * np.eye(N)
+    A[7][N - 1] = 0
+    A[11] = np.eye(N)[np.r_[N // 2:N, 0:N // 2]]
+    if precision:
+        A = A.astype(np.complex64).astype(np.complex128)
+    print(f"[gj layouts] N={N} precision={precision}", flush=True)
+    i1, l1, f1 = gj_inverse_layout(A, 1, precision)
+    i2, l2, f2 = gj_inverse_layout(A, 2, precision)
+    assert np.array_equal(f1, f2) and f2[7] == 1 and np.count_nonzero(f2) == 1
+    assert l2[7].real == -1e300
+    ok = [b for b in range(batch) if b != 7]
+    assert np.max(np.abs(l2[ok] - l1[ok])) < 1e-13 * N
+    assert np.array_equal(i2[11], A[11].T)
+    scale = np.max(np.abs(i1[ok]), axis=(1, 2))
+    rel = np.max(np.abs(i2[ok] - i1[ok]), axis=(1, 2)) / scale
+    assert np.max(rel) < (1e-6 if precision else 1e-12)
+    tol = 1e-4 if precision else 1e-11 * N
+    for b in [b for b in range(0, batch, 29) if b not in (7, 11)]:
+        assert np.max(np.abs(i2[b] @ A[b] - np.eye(N))) < tol, b
 
 
 def test_gj_pivoting_cases():
