@@ -7,6 +7,7 @@ per-walker loop replaced by batched device calls that keep walkers, weights,
 ET and the random counter in HBM.
 """
 import ctypes as C
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -66,6 +67,10 @@ class Context:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -90,6 +95,10 @@ class Graph:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -134,6 +143,10 @@ class CPGroup:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -246,6 +259,10 @@ class CPPropagator:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -423,6 +440,10 @@ class WalkerSet:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -615,6 +636,10 @@ class BackPropagation:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

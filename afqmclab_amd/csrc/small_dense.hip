@@ -20,6 +20,9 @@
 //   second row group in LDS: up to 128 registers and 68 KB of LDS per matrix let two matrices
 //   share a CU, and each hides the other's per-step synchronisation.  Up to 64 rows, 8 or 4 waves per matrix
 //   (one row group in registers) let several matrices share a CU.
+//   Two more kernels give the same pivots: a 2-D lane layout (both operands of each rank-1 update
+//   from LDS) and a kernel in panels of four columns (rank-4 updates, with one wave pivoting the
+//   next panel while the others update); the dispatcher picks by size.
 // chol_inverse: S = R^H R (upper R, positive real diagonal) and R^{-1} in one sweep, row
 //   formulation in one array: after the Jacobi scaling S' = D^-1/2 S D^-1/2, step k leaves
 //   row k of the working array unscaled (scale r_kk: R's row in the upper part, the row of
@@ -583,6 +586,337 @@ __device__ __forceinline__ void Gj2K<T, WV, LR, NR, NC, WPEV>::run(const BlockId
   }
 }
 
+// Gauss-Jordan in panels of four columns.  The exchange steps of one panel compose to a left
+// factor L that differs from the identity only in the columns of the panel's pivot rows P, so every
+// column outside the panel becomes A - M A[P, :] with M = (I - L)[:, P] = S + I[:, P], S being the
+// panel's own columns after its four in-place steps: a panel costs one rank-4 update and one
+// barrier instead of four rank-1 steps with a barrier each.
+// One wave (w = WV) does all the pivoting: it factorises panel j+1 while the others apply panel j.
+// The update waves hand it each panel's columns one phase early (current through panel j-1), it
+// applies panel j to them itself, and its serial pivot chain (lanes = rows, NRF row groups, so the
+// search is a wave reduction) overlaps the bulk of the update.  Update wave w holds the CPW = 4 NC
+// columns CPW w .. CPW w + CPW-1 of all rows, in a 2-D layout: lane lr + 16 lc holds rows lr + 16r
+// (r < NR) of columns CPW w + lc + 4c (c < NC).  The pivot rows of its own columns are then in its
+// own lanes, which write them to the wave's LDS row buffer for the others to read back, and panel
+// b is register column b % NC of wave b / NC.  Per panel and lane: 4 NR NC complex multiply-adds
+// against 4 (NR + NC) LDS reads of M and of the pivot rows.
+// Pivots follow the rule of the unblocked kernels (largest |re|+|im| among unused rows, first index
+// on ties) and the row scaling is deferred the same way; results agree with them to rounding (the
+// rank-4 updates sum in another order).
+template <typename T, int NR, int NC, int WV, int NRF, int WPEV>
+struct GjBK {
+  static constexpr int NT = 64 * (WV + 1), WPE = WPEV, WPE_MAX = WPEV;
+  __device__ static void run(const BlockIdx3& bk, const SmallDenseArgs& p);
+};
+template <typename T, int NR, int NC, int WV, int NRF, int WPEV>
+__device__ __forceinline__ void GjBK<T, NR, NC, WV, NRF, WPEV>::run(const BlockIdx3& bk, const SmallDenseArgs& p) {
+  constexpr int NB = 4, CPW = 4 * NC;  // panel size, columns per update wave
+  constexpr int COLS = CPW * WV, FROWS = 64 * NRF;
+  static_assert(16 * NR <= FROWS && NR * NC >= NRF * NB, "row capacity");
+  const int bidx = bk.x;
+  if (p.alive && !p.alive[bidx / p.alive_div]) return;
+  const int N = p.N;
+  const int NP = (N + NB - 1) / NB;
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lc = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  typedef cplx<T> ct;
+  const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
+  ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
+
+  __shared__ zd pc[2][NB][FROWS];   // panel columns handed to the pivoting wave
+  __shared__ zd mm[2][NB][FROWS];   // multipliers M of a factorised panel
+  __shared__ int pp[2][NB];         // its pivot rows
+  __shared__ zd rb[WV][NB][CPW];    // each update wave's pivot rows of its own columns
+  __shared__ zd fr[NRF][NB];        // the pivoting wave's pivot row
+  __shared__ zd pivs[COLS];
+  __shared__ zd rowscale[FROWS];
+  __shared__ int perm[COLS];
+  __shared__ int qinv[FROWS];
+  __shared__ int singular, parity;
+  if (threadIdx.x == 0) { singular = 0; parity = 0; }
+
+  const bool upd = wv < WV;
+  // update waves: x[r][c] = element (lr + 16r, CPW wv + lc + 4c); panel b is register column
+  // b % NC of wave b / NC (lane column lc holding its column lc)
+  zd x[NR][NC];
+  // A register column chosen at run time is reached through independent branches on the bits of
+  // a mask the compiler cannot see through: branches on b % 4 == C would be recognised as
+  // exclusive and merged into one with the register read through a run-time pointer.
+  auto col_mask = [](int b) __attribute__((always_inline)) {
+    unsigned m = (unsigned)__builtin_amdgcn_readfirstlane(1 << (b % NC));
+    asm volatile("" : "+s"(m));
+    return m;
+  };
+  auto hand_over = [&](int b, int buf) __attribute__((always_inline)) {
+    if (b / NC != wv) return;
+    const unsigned sel = col_mask(b);
+    static_for<NC>([&](auto cc) __attribute__((always_inline)) {
+      constexpr int C = decltype(cc)::value;
+      if (sel & (1u << C)) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) pc[buf][lc][lr + 16 * r] = x[r][C];
+      }
+    });
+  };
+  if (upd) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int i = lr + 16 * r, j = CPW * wv + lc + 4 * c;
+        x[r][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
+      }
+    hand_over(0, 0);
+    if (NP > 1) hand_over(1, 1);
+  }
+
+  // pivoting wave: the multipliers of the last panel it factorised are kept in x (slot r NB + q),
+  // its pivot rows in pk
+  auto fm = [&](auto sc) -> zd& {
+    constexpr int S = decltype(sc)::value;
+    return x[S / NC][S % NC];
+  };
+  int pk[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) pk[q] = 0;
+  unsigned used = 0u;  // bit r: row lane + 64r has been a pivot row
+
+  // Factorise panel jn from its handed-over copy, after applying panel jn-1 to it when pre.
+  auto factor = [&](int jn, bool pre) __attribute__((always_inline)) {
+    const int buf = jn & 1;
+    zd y[NRF][NB];
+#pragma unroll
+    for (int r = 0; r < NRF; ++r)
+#pragma unroll
+      for (int q = 0; q < NB; ++q) y[r][q] = pc[buf][q][lane + 64 * r];
+    if (pre) {
+      // y -= M_{jn-1} y[P_{jn-1}, :]; the pivot rows come from the unmodified copy in LDS
+      static_for<NB>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int K = decltype(kc)::value;
+        zd rv[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) rv[q] = pc[buf][q][pk[K]];
+        static_for<NRF>([&](auto rc) __attribute__((always_inline)) {
+          constexpr int R = decltype(rc)::value;
+          const zd mk = fm(std::integral_constant<int, R * NB + K>{});
+#pragma unroll
+          for (int q = 0; q < NB; ++q) y[R][q] = cfms(y[R][q], mk, rv[q]);
+        });
+      });
+    }
+    int pkn[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int col = NB * jn + q;
+      pkn[q] = 0;
+      if (col >= N) continue;
+      unsigned long long key = 0ull;
+#pragma unroll
+      for (int r = 0; r < NRF; ++r) {
+        const int i = lane + 64 * r;
+        if (i < N && !((used >> r) & 1u)) {
+          const unsigned long long kk = pivot_key(y[r][q], i);
+          key = kk > key ? kk : key;
+        }
+      }
+      key = wave_max_u64(key);
+      const int bi = 255 - (int)(key & 0xFFull);
+      const int pr = (bi >= 0 && bi < N) ? bi : 0;
+      const int rr = pr >> 6, pl = pr & 63;
+      // the pivot row through LDS: its lane writes all its row groups and every lane reads group
+      // rr back (a choice among the registers themselves, y[rr], would be compiled into a read
+      // through a run-time pointer and move y out of registers)
+      if (lane == pl) {
+#pragma unroll
+        for (int r = 0; r < NRF; ++r)
+#pragma unroll
+          for (int q2 = 0; q2 < NB; ++q2) fr[r][q2] = y[r][q2];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      zd prv[NB];
+#pragma unroll
+      for (int q2 = 0; q2 < NB; ++q2) prv[q2] = fr[rr][q2];
+      const zd piv = prv[q];
+      const bool nz = abs1(piv) > 0.0;
+      const double id = nz ? recip(piv.re * piv.re + piv.im * piv.im) : 0.0;
+      const zd inv = cmk<double>(piv.re * id, -piv.im * id);
+      zd m[NRF];
+#pragma unroll
+      for (int r = 0; r < NRF; ++r) m[r] = (lane + 64 * r == pr) ? cmk<double>(0, 0) : cmul(y[r][q], inv);
+#pragma unroll
+      for (int q2 = 0; q2 < NB; ++q2) {
+        if (q2 == q) continue;
+#pragma unroll
+        for (int r = 0; r < NRF; ++r) y[r][q2] = cfms(y[r][q2], m[r], prv[q2]);
+      }
+#pragma unroll
+      for (int r = 0; r < NRF; ++r) y[r][q] = (lane + 64 * r == pr) ? cmk<double>(-1.0, 0.0) : m[r];
+      if (lane == pl) used |= 1u << rr;
+      if (lane == 0) {
+        pivs[col] = piv;
+        perm[col] = pr;
+        rowscale[pr] = cmk<double>(-inv.re, -inv.im);
+        if (!nz) singular = 1;
+      }
+      pkn[q] = pr;
+      // the next step's pivot-row write must not overtake this step's reads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // M = S + I[:, P], published and kept for the next panel (zero columns past N)
+    static_for<NB>([&](auto qc) __attribute__((always_inline)) {
+      constexpr int Q = decltype(qc)::value;
+      const bool live = NB * jn + Q < N;
+      static_for<NRF>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int R = decltype(rc)::value;
+        const int i = lane + 64 * R;
+        zd v = live ? y[R][Q] : cmk<double>(0, 0);
+        if (live && i == pkn[Q]) v.re += 1.0;
+        fm(std::integral_constant<int, R * NB + Q>{}) = v;
+        mm[buf][Q][i] = v;
+      });
+      pk[Q] = pkn[Q];
+      if (lane == 0) pp[buf][Q] = pkn[Q];
+    });
+  };
+
+  // the pivoting wave's chain is the critical path: its instructions go first on its SIMD
+  if (!upd) __builtin_amdgcn_s_setprio(2);
+  __syncthreads();
+  if (!upd) factor(0, false);
+  __syncthreads();
+  for (int j = 0; j < NP; ++j) {
+    const int buf = j & 1;
+    if (upd) {
+      int P[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) P[k] = __builtin_amdgcn_readfirstlane(pp[buf][k]);
+      // the pivot rows of this wave's columns, written by the lanes holding them: one
+      // independent store per row register, the lane condition naming the register (branches
+      // that differ only in the register would be merged into a read through a run-time pointer)
+      // (also for the padding columns of the last panel, whose pivot row 0 keeps the buffer
+      // finite under their zero multipliers)
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        static_for<NR>([&](auto rc) __attribute__((always_inline)) {
+          constexpr int R = decltype(rc)::value;
+          if (lr + 16 * R == P[k]) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) rb[wv][k][lc + 4 * c] = x[R][c];
+          }
+        });
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // x -= M_j rb, one k at a time (straight-line code, so that the loads of k+1 are issued
+      // under the multiply-adds of k; past N the multipliers are zero)
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        zd mk[NR], rk[NC];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) mk[r] = mm[buf][k][lr + 16 * r];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) rk[c] = rb[wv][k][lc + 4 * c];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) x[r][c] = cfms(x[r][c], mk[r], rk[c]);
+      }
+      // panel j's own columns become S = M - I[:, P] (zero past N)
+      if (j / NC == wv) {
+        const unsigned sel = col_mask(j);
+        static_for<NC>([&](auto cc) __attribute__((always_inline)) {
+          constexpr int C = decltype(cc)::value;
+          if (sel & (1u << C)) {
+            const int pq = pp[buf][lc];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              zd v = mm[buf][lc][lr + 16 * r];
+              if (lr + 16 * r == pq) v.re -= 1.0;
+              x[r][C] = (NB * j + lc < N) ? v : cmk<double>(0, 0);
+            }
+          }
+        });
+      }
+      if (j + 2 < NP) hand_over(j + 2, buf);
+      // this phase's reads of the row buffer before the next phase's writes
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else if (j + 1 < NP) {
+      factor(j + 1, true);
+    }
+    __syncthreads();
+  }
+
+  if (!upd) __builtin_amdgcn_s_setprio(0);
+  for (int i = threadIdx.x; i < N; i += NT) qinv[perm[i]] = i;
+  // sign of the row permutation: parity of its inversions, counted in parallel
+  {
+    int odd = 0;
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+      const int i = e / N, j = e - i * N;
+      odd ^= (i < j && perm[i] > perm[j]) ? 1 : 0;
+    }
+    if (odd) atomicXor(&parity, 1);
+  }
+  // log|det| and the phase of det from the pivots: wave 0, fixed-order trees (the 1-D kernel's)
+  double lognorm = 0.0;
+  zd phase = cmk<double>(1.0, 0.0);
+  if (wv == 0) {
+#pragma unroll
+    for (int q = 0; q < (COLS + 63) / 64; ++q) {
+      const int k = lane + 64 * q;
+      if (k < N) {
+        const zd pv = pivs[k];
+        const double ap = cabs(pv);
+        const double ia = ap > 0.0 ? 1.0 / ap : 0.0;
+        lognorm += log(ap);
+        phase = cmul(phase, cmk<double>(pv.re * ia, pv.im * ia));
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lognorm += __shfl_xor(lognorm, o, 64);
+      const zd q = cmk<double>(__shfl_xor(phase.re, o, 64), __shfl_xor(phase.im, o, 64));
+      phase = cmul(phase, q);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const zd ph = parity ? cmk<double>(-phase.re, -phase.im) : phase;
+    zd ld;
+    if (singular) {
+      ld = cmk<double>(-1e300, 0.0);
+    } else {
+      ld = cmk<double>(lognorm, atan2(ph.im, ph.re));
+      ld.re += log(hypot(ph.re, ph.im));
+    }
+    p.logdet[bidx] = ld;
+    if (p.info) p.info[bidx] = singular;
+  }
+  if (!p.write_inverse || !upd) return;
+  // inverse: O^{-1}[qinv[i]][perm[j]] = x[i][j] / (-pivot of row i)
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int i = lr + 16 * r;
+    if (i >= N) continue;
+    const int row = qinv[i];
+    const zd sc = rowscale[i];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int j = CPW * wv + lc + 4 * c;
+      if (j >= N) continue;
+      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(x[r][c], sc));
+    }
+  }
+}
+
 // Cholesky and R^{-1}, row layout: y[c][r] = element (wave + 16c, lane + 64r).
 template <typename T, int NR, int NC>
 struct CholK {
@@ -812,6 +1146,10 @@ template <typename T, int WV, int LR, int NR, int NC, int WPE>
 int run_gj2(const SmallDenseArgs& a, hipStream_t s) {
   return glaunch<Gj2K<T, WV, LR, NR, NC, WPE>>(a, dim3(a.batch), dim3(64 * WV), 0, s);
 }
+template <typename T, int NR, int NC, int WV, int NRF, int WPE>
+int run_gjb(const SmallDenseArgs& a, hipStream_t s) {
+  return glaunch<GjBK<T, NR, NC, WV, NRF, WPE>>(a, dim3(a.batch), dim3(64 * (WV + 1)), 0, s);
+}
 template <typename T, int NR, int NC>
 int run_chol(const SmallDenseArgs& a, hipStream_t s) {
   return glaunch<CholK<T, NR, NC>>(a, dim3(a.batch), dim3(64 * WAVES), 0, s);
@@ -823,12 +1161,31 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
   REQUIRE(a.alive_div >= 1, "alive_div must be >= 1");
   if (a.batch <= 0) return 0;
   if (kind == 0) {
-    // The 2-D layout by default only up to 32 columns, where it measured faster on MI355X (3072
-    // inversions of 24 x 24: 0.070 against 0.101 ms).  From 33 columns on it is slower (40: 0.40
-    // against 0.26 ms; 112: 4.55 against 2.38 ms for 4096): its registers allow two waves per SIMD,
-    // and the LDS round trips of the multipliers and the pivot row sit on every step's chain.
-    const int layout = a.layout ? a.layout : (a.N <= 32 ? 2 : 1);
-    REQUIRE(layout == 1 || layout == 2, "Gauss-Jordan layout %d unknown", layout);
+    // Defaults, from 4096 (112, 96 columns), 2048 (64), 1024 (72) and 3072 (40, 24) inversions on
+    // MI355X: the 2-D layout up to 32 columns (24: 0.070 ms against 0.101 for the 1-D layout; from
+    // 33 columns on it is slower, its registers allowing two waves per SIMD with the LDS round
+    // trips of every step on the chain); the panel kernel for 33 to 48 columns (40: 0.148 against
+    // 0.262 ms, four matrices per CU) and 65 to 112 (112: 2.16 against 2.34 ms; 72: 0.309 against
+    // 0.343; 96: 1.72 against 1.74), where its registers allow one or two matrices per CU; the 1-D
+    // layout otherwise (64: 0.29 against 0.42 for the panel kernel, which fits two matrices per CU
+    // there against the 1-D kernel's four).  The choice depends on N only, so that a grouped call
+    // and the members' own calls do the same arithmetic (the panel kernel's results differ from the
+    // others' in rounding).
+    // a grouped call launches the matrices of all its members together
+    const bool many = a.batch > cu_count() || grouping();
+    const bool panels = a.N <= 48 || (a.N > 64 && a.N <= 112);
+    const int layout = a.layout ? a.layout : a.N <= 32 ? 2 : panels ? 3 : 1;
+    REQUIRE(layout >= 1 && layout <= 3, "Gauss-Jordan layout %d unknown", layout);
+    if (layout == 3 && a.N <= 112) {
+      // ceil(N / 16) update waves of 16 columns and one pivoting wave
+      if (a.N <= 16) return run_gjb<T, 1, 4, 1, 1, 4>(a, s);
+      if (a.N <= 32) return run_gjb<T, 2, 4, 2, 1, 4>(a, s);
+      if (a.N <= 48) return run_gjb<T, 3, 4, 3, 1, 4>(a, s);
+      if (a.N <= 64) return run_gjb<T, 4, 4, 4, 1, 3>(a, s);
+      if (a.N <= 80) return run_gjb<T, 5, 4, 5, 2, 2>(a, s);
+      if (a.N <= 96) return run_gjb<T, 6, 4, 6, 2, 2>(a, s);
+      return run_gjb<T, 7, 4, 7, 2, 2>(a, s);
+    }
     if (layout == 2 && a.N <= 128) {
       if (a.N <= 32) return run_gj2<T, 2, 32, 1, 8, 4>(a, s);
       if (a.N <= 64) return run_gj2<T, 4, 32, 2, 8, 2>(a, s);
@@ -839,8 +1196,6 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
     // up to 64 columns, fewer waves per matrix and more matrices per CU: 8 waves, or 4 once
     // there are more matrices than CUs (2048 x 64^2: 0.28 against 0.36 ms with 16 waves;
     // 1024 x 40^2: 0.10 against 0.12 ms)
-    // a grouped call launches the matrices of all its members together
-    const bool many = a.batch > cu_count() || grouping();
     if (a.N <= 32) return many ? run_gj<T, 1, 8, 4, 8>(a, s) : run_gj<T, 1, 4, 8, 4>(a, s);
     if (a.N <= 64) return many ? run_gj<T, 1, 16, 4, 16>(a, s) : run_gj<T, 1, 8, 8, 8>(a, s);
     if (a.N <= 96) return many ? run_gj<T, 2, 12, 8, 6>(a, s) : run_gj<T, 2, 6>(a, s);

@@ -12,7 +12,8 @@ struct SmallDenseArgs {
   const int* alive;    // optional: skip matrix b if alive[b / alive_div] == 0
   int alive_div;
   double pivot_tol;    // chol only: info = 1 also when a pivot of the Jacobi-scaled matrix is <= this
-  int layout;          // gj only: 0 the default kernel for N, 1 the 1-D lane layout, 2 the 2-D lane layout
+  int layout;          // gj only: 0 the default kernel for N, 1 the 1-D lane layout, 2 the 2-D lane layout,
+                       // 3 panels of four columns (N <= 112)
 };
 
 template <typename T> int launch_gj_inverse(const SmallDenseArgs& a, hipStream_t s);

@@ -6,6 +6,7 @@ step order, cadences, background-adjust schedule and random-number consumption
 order, with the per-walker loop replaced by batched device calls.
 """
 import ctypes as C
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -179,6 +180,10 @@ class PhaselessPropagator:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -320,6 +325,10 @@ class WalkerSet2is:
             self.h = None
 
     def __del__(self):
+        # no native calls while the interpreter shuts down (the runtime may already be tearing
+        # down; the process exit reclaims the memory)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -382,8 +382,10 @@ int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K,
 /* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting) */
 int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,
                           double* logdet, int* info);
-/* the same with the kernel's lane layout chosen: 1 lanes = rows (readlanes for the pivot row), 2 a 2-D
- * lane layout with both update operands from LDS (N <= 128); both give identical results */
+/* the same with the kernel chosen: 1 lanes = rows (readlanes for the pivot row), 2 a 2-D lane layout
+ * with both update operands from LDS (N <= 128), identical to 1; 3 panels of four columns with rank-4
+ * updates (N <= 112), equal to 1 up to rounding (same pivots, updates summed in another order);
+ * 0 the default for N and batch */
 int afqmc_test_gj_inverse_layout(afqmc_ctx ctx, int precision, int N, int batch, int layout, const double* A,
                                  double* Ainv, double* logdet, int* info);
 /* device-to-device streaming copy of `bytes` (a multiple of 16), reps times: the reference kernel

@@ -68,7 +68,8 @@ def gj_inverse(A, precision=0):
 
 
 def gj_inverse_layout(A, layout, precision=0):
-    """gj_inverse with the kernel's lane layout chosen (1: lanes = rows, 2: 2-D lane layout)."""
+    """gj_inverse with the kernel chosen (1: lanes = rows, 2: 2-D lane layout, 3: panels of four
+    columns)."""
     A = np.asarray(A, dtype=np.complex128)
     batch, N, _ = A.shape
     buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))

@@ -213,6 +213,42 @@ def test_gj_two_d_layout_matches_one_d(N, precision):
         assert np.max(np.abs(i2[b] @ A[b] - np.eye(N))) < tol, b
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("N", [1, 3, 4, 5, 13, 32, 33, 40, 63, 64, 65, 72, 96, 97, 100, 112])
+def test_gj_panel_kernel_matches_one_d(N, precision):
+    """The Gauss-Jordan kernel in panels of four columns (rank-4 updates, the next panel factorised
+    by its own wave during the update) against the unblocked 1-D kernel on the same matrices, with
+    a singular and a permutation matrix among them: the same breakdown flags, log-determinants to
+    1e-12 N, inverses to 1e-11 relative (FP64) or FP32 rounding, the permutation's inverse exact,
+    and inv A = I.  The two sum the updates in different orders, so they agree to rounding only."""
+    rng = np.random.default_rng(70 + N + precision)
+    batch = 300
+    A = crand(rng, batch, N, N) + 0.5 * np.eye(N)
+    A[7][N - 1] = 0
+    A[11] = np.eye(N)[np.r_[N // 2:N, 0:N // 2]]
+    if precision:
+        A = A.astype(np.complex64).astype(np.complex128)
+    print(f"[gj panels] N={N} precision={precision}", flush=True)
+    i1, l1, f1 = gj_inverse_layout(A, 1, precision)
+    i3, l3, f3 = gj_inverse_layout(A, 3, precision)
+    assert np.array_equal(f1, f3) and f3[7] == 1 and np.count_nonzero(f3) == 1
+    assert l3[7].real == -1e300
+    ok = [b for b in range(batch) if b != 7]
+    assert np.max(np.abs(l3[ok] - l1[ok])) < 1e-12 * N
+    assert np.array_equal(i3[11], A[11].T)
+    scale = np.max(np.abs(i1[ok]), axis=(1, 2))
+    rel = np.max(np.abs(i3[ok] - i1[ok]), axis=(1, 2)) / scale
+    assert np.max(rel) < (1e-5 if precision else 1e-11)
+    tol = 1e-4 if precision else 1e-11 * N
+    for b in [b for b in range(0, batch, 29) if b not in (7, 11)]:
+        assert np.max(np.abs(i3[b] @ A[b] - np.eye(N))) < tol, b
+    sign, logabs = np.linalg.slogdet(A[ok[:20]])
+    ref = logabs + 1j * np.angle(sign)
+    d = l3[ok[:20]] - ref
+    d = d.real + 1j * ((d.imag + np.pi) % (2 * np.pi) - np.pi)
+    assert np.max(np.abs(d)) < 1e-10 * N
+
+
 def test_gj_pivoting_cases():
     # permutation matrix (one swap -> det -1), exactly singular matrix, identity
     P = np.eye(8, dtype=complex)[[1, 0, 2, 3, 4, 5, 6, 7]]
