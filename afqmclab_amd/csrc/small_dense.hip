@@ -1181,7 +1181,7 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
       if (a.N <= 16) return run_gjb<T, 1, 4, 1, 1, 4>(a, s);
       if (a.N <= 32) return run_gjb<T, 2, 4, 2, 1, 4>(a, s);
       if (a.N <= 48) return run_gjb<T, 3, 4, 3, 1, 4>(a, s);
-      if (a.N <= 64) return run_gjb<T, 4, 4, 4, 1, 3>(a, s);
+      if (a.N <= 64) return run_gjb<T, 4, 4, 4, 1, 4>(a, s);
       if (a.N <= 80) return run_gjb<T, 5, 4, 5, 2, 2>(a, s);
       if (a.N <= 96) return run_gjb<T, 6, 4, 6, 2, 2>(a, s);
       return run_gjb<T, 7, 4, 7, 2, 2>(a, s);
