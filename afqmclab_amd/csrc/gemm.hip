@@ -363,14 +363,17 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
     const ct* P2 = reinterpret_cast<const ct*>(p.P2);
     const bool use1 = p.partner_half != 0, use2 = P2 != nullptr;
     __shared__ zd red[3][WN][BM];
-    zd d0[FM][4], d1[FM][4], d2[FM][4];
+    // each wave's dots over its 32 columns in the accumulators' precision (FP32 tiles: 32 terms
+    // per partial, far below the mixed path's error), the sums across waves and tiles in FP64
+    typedef cplx<T> dt;
+    dt d0[FM][4], d1[FM][4], d2[FM][4];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        d0[i][r] = cmk<double>(0, 0);
-        d1[i][r] = cmk<double>(0, 0);
-        d2[i][r] = cmk<double>(0, 0);
+        d0[i][r] = cmk<T>(0, 0);
+        d1[i][r] = cmk<T>(0, 0);
+        d2[i][r] = cmk<T>(0, 0);
       }
     // FP32 accumulators: lane (lk, lr) holds rows 4 lk .. 4 lk + 3 of column lr, which are
     // contiguous in a column-major partner: two 16-byte loads instead of four scattered ones
@@ -415,10 +418,10 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
             if (use2) load4(P2, gm0, gn, vec_ok, q2);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              zd x = cmk<double>((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
-              d0[i][r] = cadd(d0[i][r], cmul(x, cconj(to_zd(q0[r]))));
-              if (use1) d1[i][r] = cadd(d1[i][r], cmul(x, cconj(to_zd(q1[r]))));
-              if (use2) d2[i][r] = cadd(d2[i][r], cmul(x, to_zd(q2[r])));
+              const dt x = cmk<T>(acc_re[i][j][r], acc_im[i][j][r]);
+              d0[i][r] = cadd(d0[i][r], cmul(x, cconj(q0[r])));
+              if (use1) d1[i][r] = cadd(d1[i][r], cmul(x, cconj(q1[r])));
+              if (use2) d2[i][r] = cadd(d2[i][r], cmul(x, q2[r]));
             }
             continue;
           }
@@ -427,16 +430,16 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
         for (int r = 0; r < 4; ++r) {
           int gm = m0 + wm * TM + i * 16 + mfma_traits<T>::row(lk, r);
           if (gm < p.M && gn < p.N) {
-            zd x = cmk<double>((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
-            zd q0 = to_zd(P0[gm + (long long)gn * p.ldp]);
+            const dt x = cmk<T>(acc_re[i][j][r], acc_im[i][j][r]);
+            const dt q0 = P0[gm + (long long)gn * p.ldp];
             d0[i][r] = cadd(d0[i][r], cmul(x, cconj(q0)));
             if (use1) {
               int gp = gm < p.partner_half ? gm + p.partner_half : gm - p.partner_half;
-              zd q1 = to_zd(P0[gp + (long long)gn * p.ldp]);
+              const dt q1 = P0[gp + (long long)gn * p.ldp];
               d1[i][r] = cadd(d1[i][r], cmul(x, cconj(q1)));
             }
             if (use2) {
-              zd q2 = to_zd(P2[gm + (long long)gn * p.ldp]);
+              const dt q2 = P2[gm + (long long)gn * p.ldp];
               d2[i][r] = cadd(d2[i][r], cmul(x, q2));
             }
           }
@@ -468,9 +471,9 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
         }
         if (lr == 0) {
           int m = wm * TM + i * 16 + mfma_traits<T>::row(lk, r);
-          red[0][wn][m] = d0[i][r];
-          red[1][wn][m] = d1[i][r];
-          red[2][wn][m] = d2[i][r];
+          red[0][wn][m] = to_zd(d0[i][r]);
+          red[1][wn][m] = to_zd(d1[i][r]);
+          red[2][wn][m] = to_zd(d2[i][r]);
         }
       }
     __syncthreads();
