@@ -507,7 +507,7 @@ __device__ __forceinline__ void CgemmK<T, OPA, BM, BN, BK, WM, WN, EPI, SCALE>::
 
 // Split-BF16 complex GEMM, op N, plain store (see GemmArgs::split_bf16): 128 x 64 output tile, 4
 // waves of 64 x 32, K tiles of 16 (one v_mfma_f32_16x16x16bf16_1k step).  The 3M form as in
-// cgemm_kernel: P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B), each real
+// CgemmK: P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B), each real
 // product from three BF16 products (lo*hi, hi*lo, hi*hi).  Each thread loads runs of four
 // consecutive k of one row (A) or column (B), splits them once (re, im and re + im, each into hi
 // and lo BF16 parts) and stores six packed 8-byte planes, k contiguous; a lane then reads each MFMA

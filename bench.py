@@ -454,7 +454,7 @@ def run_bp_leg(ctx, model, phi, nup, nw, nbp, skip, nmeas, seed, tiles=None):
 
 
 def run_hs_bandwidth(ctx, nw, steps=6):
-    """The spin-mixing HS kernel (2x2 site matrices, hs_apply_2x2_kernel) on dense 512 x 160 walkers
+    """The spin-mixing HS kernel (2x2 site matrices, HsApplyK) on dense 512 x 160 walkers
     (80 + 80 electrons; dense walkers hold at most 160 columns) of the 16x16 lattice: `steps`
     applications of random near-identity site matrices through afqmc_cp_hs_apply, each launch timed
     with HIP events on the context stream, against the bytes it must move (read W, write W', the
