@@ -1082,8 +1082,8 @@ int afqmc_cp_adjust_ET(afqmc_cp cp) {
 
 int afqmc_cp_set_tiles(afqmc_cp cp, const int* tiles) {
   REQUIRE(cp && tiles, "null argument");
-  for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= 0 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
-  REQUIRE(gemm_tile_green(tiles[2]), "the Green GEMM takes tile shape 0, 1, 4, 5, 7, 8 or 9");
+  for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= -1 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
+  REQUIRE(tiles[2] < 0 || gemm_tile_green(tiles[2]), "the Green GEMM takes tile shape 0, 1, 4, 5, 7, 8 or 9");
   for (int r = 0; r < 3; ++r) cp->tile[r] = tiles[r];
   return 0;
 }
@@ -1134,7 +1134,7 @@ int run_grouped(afqmc_cp_group g, F&& per_member) {
   for (size_t i = 0; i < n; ++i)
     for (int r = 0; r < 3; ++r) {
       saved[i][r] = g->cps[i]->tile[r];
-      g->cps[i]->tile[r] = g->tile[r];
+      if (g->tile[r] >= 0) g->cps[i]->tile[r] = g->tile[r];  // -1: the member's own shape
     }
   g_group = &c;
   int rc = 0;
@@ -1182,8 +1182,8 @@ int afqmc_cp_group_destroy(afqmc_cp_group g) {
 
 int afqmc_cp_group_set_tiles(afqmc_cp_group g, const int* tiles) {
   REQUIRE(g && tiles, "null argument");
-  for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= 0 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
-  REQUIRE(gemm_tile_green(tiles[2]), "the Green GEMM takes tile shape 0, 1, 4, 5, 7, 8 or 9");
+  for (int r = 0; r < 3; ++r) REQUIRE(tiles[r] >= -1 && tiles[r] < GEMM_TILE_COUNT, "tile %d out of range", tiles[r]);
+  REQUIRE(tiles[2] < 0 || gemm_tile_green(tiles[2]), "the Green GEMM takes tile shape 0, 1, 4, 5, 7, 8 or 9");
   for (int r = 0; r < 3; ++r) g->tile[r] = tiles[r];
   return 0;
 }

@@ -145,7 +145,8 @@ int afqmc_cp_autotune(afqmc_cp cp, afqmc_walkers w, int reps, int* tiles, double
  * outputs) do on every member, with the same results, but launch each kernel of a phase once over
  * all members (about ten launches per step instead of ten per member), which a graph captured from
  * group calls replays.  The group's tile shapes (set_tiles, as afqmc_cp_set_tiles; 64x64 by default)
- * stand in for the members' own during its calls.  Argument tables of the grouped launches are kept
+ * stand in for the members' own during its calls; a shape of -1 keeps each member's own for that
+ * GEMM (members with equal shapes share a launch, each other shape adds one).  Argument tables of the grouped launches are kept
  * on the device until group_destroy (table_count: how many), so graphs captured from a group's calls
  * must be destroyed before it.  After an error the members' walker sets are undefined. */
 typedef struct afqmc_cp_group_s* afqmc_cp_group;

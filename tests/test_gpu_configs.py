@@ -386,20 +386,27 @@ def test_grouped_step_matches_oracle(precision):
         prop.close()
 
 
+@pytest.mark.parametrize("own_tiles", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_grouped_graph_equals_per_instance_calls(precision):
+def test_grouped_graph_equals_per_instance_calls(precision, own_tiles):
     """Blocks of 10 grouped steps + grouped QR (+ grouped comb every other block), captured as
     graphs and replayed with eager per-instance work between the replays, against the same
     per-instance calls made eagerly on identical propagators: log-overlaps, walkers, weights and
-    alive flags identical.  The block without the comb stays within 400 graph nodes."""
+    alive flags identical.  With own_tiles the members keep different GEMM tile shapes of their
+    own (group tiles -1: one grouped launch per shape in use).  The block without the comb stays
+    within 400 graph nodes (600 with four shapes per GEMM)."""
     nw = 256
     c = Context(0)
     gsets = hetero_sets(precision, nw, [c] * 24)
     esets = hetero_sets(precision, nw, [ctx()] * 24)
+    green_shapes = [0, 1, 4, 5]
     for j, (a, b) in enumerate(zip(gsets, esets)):
         a[3].set_seed(300 + j, 0)
         b[3].set_seed(300 + j, 0)
-    grp = CPGroup([(prop, ws) for *_, prop, ws in gsets])
+        if own_tiles:
+            for s_ in (a, b):
+                s_[3].set_tiles(j % 4, (j + 1) % 4, green_shapes[j % 4])
+    grp = CPGroup([(prop, ws) for *_, prop, ws in gsets], tiles=(-1, -1, -1) if own_tiles else None)
 
     def eager_block(sets, comb):
         for *_, prop, ws in sets:
@@ -425,9 +432,9 @@ def test_grouped_graph_equals_per_instance_calls(precision):
         group_block(comb)
         graphs[comb] = c.capture_end()
     nodes = graphs[False].nodes()
-    print(f"[c4 grouped] precision={precision}: {nodes} graph nodes per 10-step block, "
-          f"{grp.table_count()} argument tables", flush=True)
-    assert nodes <= 400
+    print(f"[c4 grouped] precision={precision} own_tiles={own_tiles}: {nodes} graph nodes per 10-step "
+          f"block, {grp.table_count()} argument tables", flush=True)
+    assert nodes <= (600 if own_tiles else 400)
     for rep in range(4):
         comb = rep % 2 == 1
         graphs[comb].launch()
