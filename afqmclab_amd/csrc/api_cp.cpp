@@ -157,17 +157,21 @@ int step_impl(afqmc_cp cp, afqmc_walkers w, double* force_out, int* aux_out, int
   return 0;
 }
 
+CholQrBlock qr_block(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tmp, int b, double tol) {
+  CholQrBlock q;
+  q.nw = w->nw; q.M = cp->Mb[b]; q.N = cp->Nb[b]; q.nsub = cp->Nb[b];
+  q.wf = &wf[b]; q.tmp = &tmp[b];
+  q.S = reinterpret_cast<zd*>(cp->qrS[b].p); q.ld = cp->qrld[b].p;
+  q.info = cp->qrinfo[b].p; q.need = cp->qr_need.p; q.flag = cp->qr_flag.p;
+  q.alive = w->alive.p; q.logw = w->logw.p;
+  q.m0 = 1.0; q.m1 = 0.0; q.tol = tol; q.sc = cp->sc.p;
+  return q;
+}
+
 // SD::stabilize of each block by robust CholeskyQR in FP64 (cholqr.hpp): W <- Q, logw += sum log R_kk
 int qr_impl_f64(afqmc_cp cp, afqmc_walkers w, DevBuf<char>* wf, DevBuf<char>* tmp, double tol) {
   for (int b = 0; b < cp->nblk; ++b) {
-    CholQrBlock q;
-    q.nw = w->nw; q.M = cp->Mb[b]; q.N = cp->Nb[b]; q.nsub = cp->Nb[b];
-    q.wf = &wf[b]; q.tmp = &tmp[b];
-    q.S = reinterpret_cast<zd*>(cp->qrS[b].p); q.ld = cp->qrld[b].p;
-    q.info = cp->qrinfo[b].p; q.need = cp->qr_need.p; q.flag = cp->qr_flag.p;
-    q.alive = w->alive.p; q.logw = w->logw.p;
-    q.m0 = 1.0; q.m1 = 0.0; q.tol = tol; q.sc = cp->sc.p;
-    int rc = cholqr_f64(q, cp->ctx->stream);
+    int rc = cholqr_f64(qr_block(cp, w, wf, tmp, b, tol), cp->ctx->stream);
     if (rc) return rc;
   }
   return 0;
@@ -189,12 +193,46 @@ int stabilize_impl(afqmc_cp cp, afqmc_walkers w) {
       if ((rc = launch_convert_f32_f64(w->wf[b].p, cp->qr64[b].p, w->wsize(b) * w->nw, s))) return rc;
     }
   }
-  // the result is rounded to FP32, so a first-pass Q orthonormal to 1e-8 is final
-  if ((rc = qr_impl_f64(cp, w, cp->qr64, cp->qr64b, 1e-8))) return rc;
+  // pass 1 and the shifted walkers' further passes in FP64, then the result rounded to FP32
+  for (int b = 0; b < cp->nblk; ++b) {
+    CholQrBlock q = qr_block(cp, w, cp->qr64, cp->qr64b, b, 1e-8);
+    q.check_all = false;
+    if ((rc = cholqr_f64(q, s))) return rc;
+  }
+  {
+    GroupFork fork;
+    for (int b = 0; b < cp->nblk; ++b) {
+      fork.branch();
+      if ((rc = launch_convert_f64_f32(cp->qr64[b].p, w->wf[b].p, w->wsize(b) * w->nw, s))) return rc;
+    }
+  }
+  // Pass 2 from the rounded walkers, whose orthonormality FP32 rounding already limits to about
+  // 1e-7: their Gram matrix in FP32 (a quarter of the FP64 Gram's time), and the walkers farther
+  // than 1e-4 from orthonormal (beyond that Gram's own rounding) take the FP64 check and pass to
+  // 1e-8 on their FP64 copy and are rounded again.
   GroupFork fork;
   for (int b = 0; b < cp->nblk; ++b) {
     fork.branch();
-    if ((rc = launch_convert_f64_f32(cp->qr64[b].p, w->wf[b].p, w->wsize(b) * w->nw, s))) return rc;
+    const int N = cp->Nb[b], M = cp->Mb[b];
+    GemmArgs a;
+    memset(&a, 0, sizeof(a));
+    a.alpha_re = 1.0;
+    a.scale_group = 1;
+    a.M = N; a.N = N; a.K = M; a.batch = w->nw;
+    a.A = w->wf[b].p; a.lda = M; a.strideA = (long long)M * N;
+    a.B = w->wf[b].p; a.ldb = M; a.strideB = (long long)M * N;
+    a.C = cp->qrS[b].p; a.ldc = N; a.strideC = (long long)N * N;
+    a.alive = w->alive.p;
+    a.herm = 1;
+    a.tile = GEMM_TILE_64x64;
+    if ((rc = launch_cgemm<float>(OP_C, EPI_STORE, a, s))) return rc;
+    if ((rc = launch_ortho_check_f32(w->nw, N, reinterpret_cast<const zf*>(cp->qrS[b].p), w->alive.p, 1e-4,
+                                     cp->qr_need.p, s)))
+      return rc;
+    const CholQrBlock q = qr_block(cp, w, cp->qr64, cp->qr64b, b, 1e-8);
+    if ((rc = cholqr_refine(q, cp->qr_need.p, s))) return rc;
+    if ((rc = launch_convert_f64_f32_masked(cp->qr64[b].p, w->wf[b].p, w->wsize(b), w->nw, cp->qr_need.p, s)))
+      return rc;
   }
   return 0;
 }

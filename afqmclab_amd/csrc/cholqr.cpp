@@ -91,6 +91,8 @@ int cholqr_f64(const CholQrBlock& q, hipStream_t s) {
   std::swap(*q.wf, *q.tmp);
   if ((rc = launch_logw_add_real(q.nw, q.logw, q.alive, q.ld, nullptr, 0, q.m0, q.m1, s))) return rc;
   // pass 2: every alive walker not yet orthonormal to tol; pass 3: shifted ones still not
-  if ((rc = refine(q, q.alive, s))) return rc;
+  if (q.check_all && (rc = refine(q, q.alive, s))) return rc;
   return refine(q, q.flag, s);
 }
+
+int cholqr_refine(const CholQrBlock& q, const int* among, hipStream_t s) { return refine(q, among, s); }

@@ -17,6 +17,7 @@ struct CholQrBlock {
   double m0 = 1.0, m1 = 0.0;            // logw += m0 * sum log R_kk + m1 * sum_{k<nsub} log R_kk
   double tol = 1e-13;                   // second pass where |Q^H Q - I| exceeds this
   DevScalars* sc = nullptr;             // counts of shifted and dead walkers
+  bool check_all = true;                // pass 2 over every alive walker (else the caller's own check)
 };
 
 // W <- Q with W = Q R, R upper triangular with positive diagonal; logw += log det R (by m0, m1).
@@ -32,4 +33,9 @@ struct CholQrBlock {
 // very negative log-weight, as Householder gives it; one with an exactly zero column after the
 // shifted pass (Householder's R_kk = 0, log det R = -inf, weight zero), or with non-finite
 // entries, breaks down in a later pass and is marked dead instead.
+// With check_all false pass 2 is left to the caller, which runs cholqr_refine over the walkers its
+// own check selects.
 int cholqr_f64(const CholQrBlock& q, hipStream_t s);
+// A further CholeskyQR of the walkers in `among` whose Q^H Q is farther than tol from I (need
+// lists them afterwards); the others stay put.
+int cholqr_refine(const CholQrBlock& q, const int* among, hipStream_t s);

@@ -694,31 +694,34 @@ __global__ void site_final_kernel(EnergyArgs a) {
 }
 
 // need[w] = alive && max_ij |S_ij - delta_ij| > tol (S = Q^H Q of walker w), skip[w] = alive && !need
+template <typename T>
 struct OrthoArgs {
-  const zd* S;
+  const cplx<T>* S;
   const int* alive;
   int* need;
   double tol;
   int N, pad;
 };
+template <typename T>
 struct OrthoK {
   static constexpr int NT = 256, WPE = 1;
-  __device__ static void run(const BlockIdx3& bk, const OrthoArgs& g);
+  __device__ static void run(const BlockIdx3& bk, const OrthoArgs<T>& g);
 };
-__device__ __forceinline__ void OrthoK::run(const BlockIdx3& bk, const OrthoArgs& g) {
+template <typename T>
+__device__ __forceinline__ void OrthoK<T>::run(const BlockIdx3& bk, const OrthoArgs<T>& g) {
   const int N = g.N;
-  const zd* S = g.S;
+  const cplx<T>* S = g.S;
   const int* alive = g.alive;
   const double tol = g.tol;
   int* need = g.need;
 
   const int w = bk.x;
   __shared__ double sh[8];
-  const zd* s = S + (long long)w * N * N;
+  const cplx<T>* s = S + (long long)w * N * N;
   double m = 0.0;
   for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
     const int i = e % N, j = e / N;
-    const zd v = s[e];
+    const zd v = to_zd(s[e]);
     const double d = fabs(v.re - (i == j ? 1.0 : 0.0)) + fabs(v.im);
     m = (d > m || d != d) ? d : m;  // NaN counts as not orthonormal
   }
@@ -1145,10 +1148,17 @@ int launch_energy(const EnergyArgs& a, hipStream_t s) {
 
 int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, hipStream_t s) {
   if (nw <= 0) return 0;
-  OrthoArgs g;
+  OrthoArgs<double> g;
   memset(&g, 0, sizeof(g));
   g.S = S; g.alive = alive; g.need = need; g.tol = tol; g.N = N;
-  return glaunch<OrthoK>(g, dim3(nw), dim3(256), 0, s);
+  return glaunch<OrthoK<double>>(g, dim3(nw), dim3(256), 0, s);
+}
+int launch_ortho_check_f32(int nw, int N, const zf* S, const int* alive, double tol, int* need, hipStream_t s) {
+  if (nw <= 0) return 0;
+  OrthoArgs<float> g;
+  memset(&g, 0, sizeof(g));
+  g.S = S; g.alive = alive; g.need = need; g.tol = tol; g.N = N;
+  return glaunch<OrthoK<float>>(g, dim3(nw), dim3(256), 0, s);
 }
 
 int launch_qr_flag(int nw, const int* alive, const int* info, int* flag, hipStream_t s) {

@@ -110,6 +110,8 @@ int launch_energy(const EnergyArgs& a, hipStream_t s);
 int launch_set_et_from_energy(DevScalars* sc, hipStream_t s);
 // CholeskyQR: which walkers still need a second pass (S = Q^H Q farther than tol from I)
 int launch_ortho_check(int nw, int N, const zd* S, const int* alive, double tol, int* need, hipStream_t s);
+// the same over FP32 Gram matrices
+int launch_ortho_check_f32(int nw, int N, const zf* S, const int* alive, double tol, int* need, hipStream_t s);
 // flag[w] = alive[w] && info[w] (a Cholesky breakdown of walker w)
 int launch_qr_flag(int nw, const int* alive, const int* info, int* flag, hipStream_t s);
 // S_w += c * tr(S_w) * I for the flagged walkers (the shift of shifted CholeskyQR)

@@ -91,6 +91,30 @@ __device__ __forceinline__ void ConvertK<A, B>::run(const BlockIdx3& bk, const C
     out[i] = cmk<B>((B)in[i].re, (B)in[i].im);
 }
 
+// per walker: convert the n elements of walker w where mask[w] is set
+template <typename A, typename B>
+struct ConvertMaskedArgs {
+  const cplx<A>* in;
+  cplx<B>* out;
+  const int* mask;
+  long long n;
+};
+template <typename A, typename B>
+struct ConvertMaskedK {
+  static constexpr int NT = 256, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const ConvertMaskedArgs<A, B>& g);
+};
+template <typename A, typename B>
+__device__ __forceinline__ void ConvertMaskedK<A, B>::run(const BlockIdx3& bk, const ConvertMaskedArgs<A, B>& g) {
+  const int w = bk.y;
+  if (!g.mask[w]) return;
+  const long long n = g.n;
+  const cplx<A>* in = g.in + (long long)w * n;
+  cplx<B>* out = g.out + (long long)w * n;
+  for (long long i = (long long)bk.x * blockDim.x + threadIdx.x; i < n; i += (long long)bk.gx * blockDim.x)
+    out[i] = cmk<B>((B)in[i].re, (B)in[i].im);
+}
+
 __global__ void logw_shift_kernel(int nw, const zd* logw, zd* out, zd c) {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w < nw) out[w] = cadd(logw[w], c);
@@ -146,6 +170,13 @@ int launch_convert_f64_f32(const void* in, void* out, long long n, hipStream_t s
   if (n <= 0) return 0;
   ConvertArgs<double, float> a{(const zd*)in, (zf*)out, n};
   return glaunch<ConvertK<double, float>>(a, dim3(g), dim3(256), 0, s);
+}
+int launch_convert_f64_f32_masked(const void* in, void* out, long long per_walker, int nw, const int* mask,
+                                  hipStream_t s) {
+  if (nw <= 0 || per_walker <= 0) return 0;
+  ConvertMaskedArgs<double, float> a{(const zd*)in, (zf*)out, mask, per_walker};
+  const unsigned gx = (unsigned)std::min<long long>((per_walker + 255) / 256, 64);
+  return glaunch<ConvertMaskedK<double, float>>(a, dim3(gx, nw), dim3(256), 0, s);
 }
 int launch_logw_shift(int nw, const zd* logw, zd* out, zd c, hipStream_t s) {
   hipLaunchKernelGGL(logw_shift_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, nw, logw, out, c);

@@ -150,6 +150,9 @@ int afqmc_noalloc_scratch_init(int device);
 
 int launch_convert_f32_f64(const void* in, void* out, long long ncomplex, hipStream_t s);
 int launch_convert_f64_f32(const void* in, void* out, long long ncomplex, hipStream_t s);
+// the per_walker elements of each walker w with mask[w] set
+int launch_convert_f64_f32_masked(const void* in, void* out, long long per_walker, int nw, const int* mask,
+                                  hipStream_t s);
 // logw_out[w] = logw[w] + c
 int launch_logw_shift(int nw, const zd* logw, zd* logw_out, zd c, hipStream_t s);
 // logw[w] += c for alive walkers
