@@ -119,17 +119,24 @@ struct GroupRecord {
   int member, pos, seq;
 };
 
-// Device copies of argument tables, by content, for the life of the owner.
+// Device copies of argument tables, by content, for the life of the owner; and the streams and
+// events over which a position's launches of different kernels run side by side.
 class GroupTables {
  public:
+  static constexpr int FORKS = 3;  // side streams for the second and later kernels of a position
   ~GroupTables();
   // a device copy of the n bytes at p (the same pointer for the same bytes); nullptr on failure
   const void* get(const void* p, size_t n);
   size_t count() const { return map_.size(); }
+  // side stream i (default priority, non-blocking) and event i (FORKS + 1 of them); created on
+  // first use, false on failure
+  bool fork_resources(hipStream_t* streams, hipEvent_t* events);
 
  private:
   std::unordered_map<std::string, void*> map_;
   hipStream_t side_ = nullptr;
+  hipStream_t fork_[FORKS] = {};
+  hipEvent_t ev_[FORKS + 1] = {};
 };
 
 struct GroupCollector {
@@ -141,7 +148,11 @@ struct GroupCollector {
 extern thread_local GroupCollector* g_group;
 inline bool grouping() { return g_group != nullptr; }
 
-// the collector's records, launched position by position on s; clears them
+// the collector's records, launched position by position on s; clears them.  The launches of one
+// position touch disjoint buffers (different members, or the independent branches of a GroupFork):
+// the second and later kernels of a position run on side streams forked from s and joined back
+// before the next position, so that kernels of different shapes (the small-dense classes of
+// lattices of different sizes) share the GPU.
 int group_flush_all(GroupCollector& c, hipStream_t s);
 
 // Branches over independent buffers of one member: each branch restarts at the fork's position,

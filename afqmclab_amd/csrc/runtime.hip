@@ -22,6 +22,22 @@ thread_local GroupCollector* g_group = nullptr;
 GroupTables::~GroupTables() {
   for (auto& kv : map_) (void)hipFree(kv.second);
   if (side_) (void)hipStreamDestroy(side_);
+  for (hipStream_t& f : fork_)
+    if (f) (void)hipStreamDestroy(f);
+  for (hipEvent_t& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+bool GroupTables::fork_resources(hipStream_t* streams, hipEvent_t* events) {
+  for (int i = 0; i < FORKS; ++i) {
+    if (!fork_[i] && hipStreamCreateWithFlags(&fork_[i], hipStreamNonBlocking) != hipSuccess) return false;
+    streams[i] = fork_[i];
+  }
+  for (int i = 0; i <= FORKS; ++i) {
+    if (!ev_[i] && hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming) != hipSuccess) return false;
+    events[i] = ev_[i];
+  }
+  return true;
 }
 
 const void* GroupTables::get(const void* p, size_t n) {
@@ -57,14 +73,35 @@ int group_flush_all(GroupCollector& c, hipStream_t s) {
     if (shape(a) != shape(b)) return shape(a) < shape(b);
     return a->seq < b->seq;
   });
+  hipStream_t fork[GroupTables::FORKS];
+  hipEvent_t ev[GroupTables::FORKS + 1];
+  bool have_forks = false;
   for (size_t i = 0; i < order.size();) {
-    size_t j = i + 1;
-    while (j < order.size() && order[j]->pos == order[i]->pos && order[j]->key == order[i]->key &&
-           shape(order[j]) == shape(order[i]))
-      ++j;
-    int rc = order[i]->flush(order.data() + i, (int)(j - i), s, c.tables);
-    if (rc) return rc;
-    i = j;
+    // the runs of one position: [i, e), each run one kernel and block shape
+    size_t e = i + 1;
+    while (e < order.size() && order[e]->pos == order[i]->pos) ++e;
+    std::vector<size_t> runs;
+    for (size_t k = i; k < e; ++k)
+      if (k == i || order[k]->key != order[k - 1]->key || shape(order[k]) != shape(order[k - 1])) runs.push_back(k);
+    runs.push_back(e);
+    const int nrun = (int)runs.size() - 1;
+    if (nrun > 1 && !have_forks) {
+      REQUIRE(c.tables->fork_resources(fork, ev), "grouped launch: no side streams");
+      have_forks = true;
+    }
+    if (nrun > 1) HIP_TRY(hipEventRecord(ev[0], s));
+    const int used = nrun > 1 ? std::min(nrun - 1, GroupTables::FORKS) : 0;
+    for (int f = 0; f < used; ++f) HIP_TRY(hipStreamWaitEvent(fork[f], ev[0], 0));
+    for (int r = 0; r < nrun; ++r) {
+      hipStream_t rs = r == 0 ? s : fork[(r - 1) % GroupTables::FORKS];
+      int rc = order[runs[r]]->flush(order.data() + runs[r], (int)(runs[r + 1] - runs[r]), rs, c.tables);
+      if (rc) return rc;
+    }
+    for (int f = 0; f < used; ++f) {
+      HIP_TRY(hipEventRecord(ev[1 + f], fork[f]));
+      HIP_TRY(hipStreamWaitEvent(s, ev[1 + f], 0));
+    }
+    i = e;
   }
   return 0;
 }
