@@ -145,6 +145,7 @@ SIGNATURES = {
                                    _dp, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "afqmc_test_gj_inverse": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip]),
     "afqmc_test_gj_inverse_layout": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip]),
+    "afqmc_test_gj_inverse_masked": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _ip]),
     "afqmc_test_copy": (C.c_int, [_vp, C.c_longlong, C.c_int, _dp]),
     "afqmc_test_chol_inverse": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _ip]),
 }

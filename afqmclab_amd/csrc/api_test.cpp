@@ -76,23 +76,27 @@ int zgemm_t(afqmc_ctx ctx, int opa, int M, int N, int K, int batch, const double
 
 template <typename T>
 int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ainv, double* logdet, int* info,
-         int layout = 0) {
+         int layout = 0, const int* alive = nullptr) {
   hipStream_t s = ctx->stream;
   const size_t n = (size_t)N * N * batch;
   auto h = narrow<T>(A, n);
   DevBuf<cplx<T>> d;
   DevBuf<zd> ld;
-  DevBuf<int> inf;
+  DevBuf<int> inf, al;
   int rc = d.alloc(n);
   if (!rc) rc = ld.alloc(batch);
   if (!rc) rc = inf.alloc(batch);
+  if (!rc && alive) rc = al.alloc(batch);
   if (rc) return rc;
   HIP_TRY(hipMemcpy(d.p, h.data(), sizeof(cplx<T>) * n, hipMemcpyHostToDevice));
+  if (alive) HIP_TRY(hipMemcpy(al.p, alive, sizeof(int) * batch, hipMemcpyHostToDevice));
+  // the outputs of skipped matrices keep these values
+  HIP_TRY(hipMemcpy(ld.p, logdet, sizeof(zd) * batch, hipMemcpyHostToDevice));
   SmallDenseArgs a;
   memset(&a, 0, sizeof(a));
   a.N = N; a.batch = batch; a.in = d.p; a.ld_in = N; a.stride_in = (long long)N * N;
   a.out = d.p; a.ld_out = N; a.stride_out = a.stride_in; a.logdet = ld.p; a.info = inf.p;
-  a.write_inverse = 1; a.nsub = N; a.alive_div = 1; a.layout = layout;
+  a.write_inverse = 1; a.nsub = N; a.alive_div = 1; a.layout = layout; a.alive = alive ? al.p : nullptr;
   rc = kind == 0 ? launch_gj_inverse<T>(a, s) : launch_chol_inverse<T>(a, s);
   if (rc) return rc;
   std::vector<cplx<T>> o(n);
@@ -136,6 +140,14 @@ int afqmc_test_gj_inverse_layout(afqmc_ctx ctx, int precision, int N, int batch,
   ScopedDevice sd(ctx);
   return precision ? gj_t<float>(ctx, 0, N, batch, A, Ainv, logdet, info, layout)
                    : gj_t<double>(ctx, 0, N, batch, A, Ainv, logdet, info, layout);
+}
+
+int afqmc_test_gj_inverse_masked(afqmc_ctx ctx, int precision, int N, int batch, int layout, const int* alive,
+                                 const double* A, double* Ainv, double* logdet, int* info) {
+  REQUIRE(ctx && alive && A && Ainv && logdet, "null argument");
+  ScopedDevice sd(ctx);
+  return precision ? gj_t<float>(ctx, 0, N, batch, A, Ainv, logdet, info, layout, alive)
+                   : gj_t<double>(ctx, 0, N, batch, A, Ainv, logdet, info, layout, alive);
 }
 
 int afqmc_test_copy(afqmc_ctx ctx, long long bytes, int reps, double* ms) {

@@ -603,9 +603,27 @@ __device__ __forceinline__ void Gj2K<T, WV, LR, NR, NC, WPEV>::run(const BlockId
 // Pivots follow the rule of the unblocked kernels (largest |re|+|im| among unused rows, first index
 // on ties) and the row scaling is deferred the same way; results agree with them to rounding (the
 // rank-4 updates sum in another order).
+// A workgroup factorises matrices bk.x, bk.x + bk.gx, ...  With two row groups (65 to 112 columns,
+// one workgroup per CU) the launch has at most one workgroup per CU, and where the matrix fits
+// LDS beside the kernel's own buffers the pivoting wave copies the workgroup's next matrix into a
+// staging buffer (direct global-to-LDS loads, 64 x 16 bytes per instruction, a few per panel)
+// while the current one is factorised: the next matrix's load is then an LDS read, and no
+// workgroup waits for another's stores to drain before it starts.
+// Workgroup barrier for LDS data only: waits for the wave's own LDS accesses, not for its
+// global loads and stores (nor for loads into LDS in flight, which a workgroup-scope fence would
+// wait for); the memory clobber keeps the compiler from moving accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// matrices per workgroup at most (the launch has at least batch / GJB_MAXM workgroups), and the
+// flags of a matrix's exponent slot: skipped (not alive), singular
+constexpr int GJB_MAXM = 32, GJB_SKIPPED = INT_MIN, GJB_SINGULAR = INT_MIN + 1;
 template <typename T, int NR, int NC, int WV, int NRF, int WPEV>
 struct GjBK {
   static constexpr int NT = 64 * (WV + 1), WPE = WPEV, WPE_MAX = WPEV;
+  static constexpr int NMAX_B = 16 * NR;
+  // (FP32 storage only: with FP64 storage at 65 to 80 columns, the one size that fits, 1024
+  // inversions of 72 x 72 took 0.309 ms against 0.295 without staging)
+  static constexpr bool STAGE = NRF == 2 && sizeof(T) == 4 && sizeof(cplx<T>) * NMAX_B * NMAX_B <= 104 * 1024;
   __device__ static void run(const BlockIdx3& bk, const SmallDenseArgs& p);
 };
 template <typename T, int NR, int NC, int WV, int NRF, int WPEV>
@@ -613,28 +631,58 @@ __device__ __forceinline__ void GjBK<T, NR, NC, WV, NRF, WPEV>::run(const BlockI
   constexpr int NB = 4, CPW = 4 * NC;  // panel size, columns per update wave
   constexpr int COLS = CPW * WV, FROWS = 64 * NRF;
   static_assert(16 * NR <= FROWS && NR * NC >= NRF * NB, "row capacity");
-  const int bidx = bk.x;
-  if (p.alive && !p.alive[bidx / p.alive_div]) return;
   const int N = p.N;
   const int NP = (N + NB - 1) / NB;
-  const int lane = threadIdx.x & 63;
-  const int lr = lane & 15, lc = lane >> 4;
+  // thread, lane and the lane's row and column in the 2-D layout: set again, opaque to the compiler,
+  // for every matrix, so that no address derived from them is hoisted out of the matrix loop into
+  // registers held across it
+  int tid = threadIdx.x, lane = tid & 63, lr = lane & 15, lc = lane >> 4;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   typedef cplx<T> ct;
-  const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
-  ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
 
-  __shared__ zd pc[2][NB][FROWS];   // panel columns handed to the pivoting wave
-  __shared__ zd mm[2][NB][FROWS];   // multipliers M of a factorised panel
-  __shared__ int pp[2][NB];         // its pivot rows
-  __shared__ zd rb[WV][NB][CPW];    // each update wave's pivot rows of its own columns
-  __shared__ zd fr[NRF][NB];        // the pivoting wave's pivot row
-  __shared__ zd pivs[COLS];
-  __shared__ zd rowscale[FROWS];
-  __shared__ int perm[COLS];
-  __shared__ int qinv[FROWS];
-  __shared__ int singular, parity;
-  if (threadIdx.x == 0) { singular = 0; parity = 0; }
+  // staging buffer: the next matrix, columns of N elements back to back, as 16-byte chunks
+  // (CPC per column); usable when every column starts on a 16-byte boundary
+  constexpr int CE = 16 / (int)sizeof(ct);  // elements per chunk
+  const bool st = STAGE && (N * (int)sizeof(ct)) % 16 == 0 && (p.ld_in * (long long)sizeof(ct)) % 16 == 0 &&
+                  (p.stride_in * (long long)sizeof(ct)) % 16 == 0 && (reinterpret_cast<uintptr_t>(p.in) & 15) == 0;
+  const int CPC = N / CE;                       // chunks per column
+  const int NI = (CPC * N + 63) / 64;           // load instructions per matrix
+  const int IPP = NP > 1 ? (NI + NP - 2) / (NP - 1) : NI;  // per panel phase of the pivoting wave
+  // the chunk of this lane in the first instruction: column q0j, chunk q0t of it
+  const int q0j = st ? lane / CPC : 0, q0t = st ? lane - q0j * CPC : 0;
+  bool staged = false;  // the staging buffer holds matrix bidx
+
+  // the workgroup's LDS as one block, the staging buffer last: the small buffers stay within
+  // reach of the 16-bit offsets of LDS instructions (16-byte aligned, for 16-byte LDS accesses)
+  struct alignas(16) Lds {
+    zd pc[2][NB][FROWS];  // panel columns handed to the pivoting wave
+    zd mm[2][NB][FROWS];  // multipliers M of a factorised panel
+    zd rb[WV][NB][CPW];   // each update wave's pivot rows of its own columns
+    zd fr[NRF][NB];       // the pivoting wave's pivot row
+    zd pivs[COLS];
+    zd rowscale[FROWS];
+    zd dz[GJB_MAXM];      // per matrix of the workgroup: det = dz 2^de (de: or a flag)
+    int de[GJB_MAXM];
+    int pp[2][NB];        // pivot rows of a factorised panel
+    int perm[COLS];
+    int qinv[FROWS];
+    int singular, parity;
+    alignas(16) ct stage[STAGE ? NMAX_B * NMAX_B : 1];
+  };
+  __shared__ Lds sm;
+  static_assert(!STAGE || sizeof(ct) * 4 * NMAX_B * WV <= sizeof(sm.pc) + sizeof(sm.mm), "output buffer");
+  auto& pc = sm.pc;
+  auto& mm = sm.mm;
+  auto& rb = sm.rb;
+  auto& fr = sm.fr;
+  auto& pivs = sm.pivs;
+  auto& rowscale = sm.rowscale;
+  auto& pp = sm.pp;
+  auto& perm = sm.perm;
+  auto& qinv = sm.qinv;
+  auto& stage = sm.stage;
+  int& singular = sm.singular;
+  int& parity = sm.parity;
 
   const bool upd = wv < WV;
   // update waves: x[r][c] = element (lr + 16r, CPW wv + lc + 4c); panel b is register column
@@ -659,17 +707,6 @@ __device__ __forceinline__ void GjBK<T, NR, NC, WV, NRF, WPEV>::run(const BlockI
       }
     });
   };
-  if (upd) {
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int i = lr + 16 * r, j = CPW * wv + lc + 4 * c;
-        x[r][c] = (i < N && j < N) ? to_zd(in[i + (long long)j * p.ld_in]) : cmk<double>(0, 0);
-      }
-    hand_over(0, 0);
-    if (NP > 1) hand_over(1, 1);
-  }
 
   // pivoting wave: the multipliers of the last panel it factorised are kept in x (slot r NB + q),
   // its pivot rows in pk
@@ -678,8 +715,6 @@ __device__ __forceinline__ void GjBK<T, NR, NC, WV, NRF, WPEV>::run(const BlockI
     return x[S / NC][S % NC];
   };
   int pk[NB];
-#pragma unroll
-  for (int q = 0; q < NB; ++q) pk[q] = 0;
   unsigned used = 0u;  // bit r: row lane + 64r has been a pivot row
 
   // Factorise panel jn from its handed-over copy, after applying panel jn-1 to it when pre.
@@ -784,136 +819,257 @@ __device__ __forceinline__ void GjBK<T, NR, NC, WV, NRF, WPEV>::run(const BlockI
     });
   };
 
-  // the pivoting wave's chain is the critical path: its instructions go first on its SIMD
-  if (!upd) __builtin_amdgcn_s_setprio(2);
-  __syncthreads();
-  if (!upd) factor(0, false);
-  __syncthreads();
-  for (int j = 0; j < NP; ++j) {
-    const int buf = j & 1;
+  // one staging load of matrix nb: chunks 64 k + lane, the lane's being chunk qt of column qj
+  auto stage_load = [&](int nb, int k, int qj, int qt) __attribute__((always_inline)) {
+    if (64 * k + lane < CPC * N) {
+      const ct* src = reinterpret_cast<const ct*>(p.in) + (long long)nb * p.stride_in + (long long)qj * p.ld_in + CE * qt;
+      __builtin_amdgcn_global_load_lds(src, stage + 64 * CE * k, 16, 0, 0);
+    }
+  };
+
+  int m = 0;  // the matrix's place in the workgroup's sequence; after the loop, their number
+  for (int bidx = bk.x; bidx < p.batch; bidx += bk.gx) {
+    if (p.alive && !p.alive[bidx / p.alive_div]) {
+      if (threadIdx.x == 0) sm.de[m] = GJB_SKIPPED;
+      staged = false;
+      ++m;
+      if (!STAGE) break;  // (without staging the launch has one workgroup per matrix)
+      continue;
+    }
+    const ct* __restrict__ in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
+    ct* __restrict__ out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
+    const int nb = bidx + bk.gx;
+    const bool stage_next = st && NP > 1 && nb < p.batch;
+    tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    lane = tid & 63;
+    lr = lane & 15;
+    lc = lane >> 4;
+    // the staging loads of this matrix (issued by the pivoting wave) have landed: vmcnt(0)
+    if (!upd && staged) __builtin_amdgcn_s_waitcnt(0x0F70);
+    // (also: every wave is done with the previous matrix's buffers)
+    lds_barrier();
+    if (tid == 0) { singular = 0; parity = 0; }
+    used = 0u;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) pk[q] = 0;
     if (upd) {
-      int P[NB];
-#pragma unroll
-      for (int k = 0; k < NB; ++k) P[k] = __builtin_amdgcn_readfirstlane(pp[buf][k]);
-      // the pivot rows of this wave's columns, written by the lanes holding them: one
-      // independent store per row register, the lane condition naming the register (branches
-      // that differ only in the register would be merged into a read through a run-time pointer)
-      // (also for the padding columns of the last panel, whose pivot row 0 keeps the buffer
-      // finite under their zero multipliers)
-#pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        static_for<NR>([&](auto rc) __attribute__((always_inline)) {
-          constexpr int R = decltype(rc)::value;
-          if (lr + 16 * R == P[k]) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) rb[wv][k][lc + 4 * c] = x[R][c];
-          }
-        });
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // x -= M_j rb, one k at a time (straight-line code, so that the loads of k+1 are issued
-      // under the multiply-adds of k; past N the multipliers are zero)
-#pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        zd mk[NR], rk[NC];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) mk[r] = mm[buf][k][lr + 16 * r];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) rk[c] = rb[wv][k][lc + 4 * c];
+      // every load issued unconditionally (element 0 in place of the padding), so that they are
+      // all in flight together
+      if (staged) {
 #pragma unroll
         for (int r = 0; r < NR; ++r)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) x[r][c] = cfms(x[r][c], mk[r], rk[c]);
-      }
-      // panel j's own columns become S = M - I[:, P] (zero past N)
-      if (j / NC == wv) {
-        const unsigned sel = col_mask(j);
-        static_for<NC>([&](auto cc) __attribute__((always_inline)) {
-          constexpr int C = decltype(cc)::value;
-          if (sel & (1u << C)) {
-            const int pq = pp[buf][lc];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-              zd v = mm[buf][lc][lr + 16 * r];
-              if (lr + 16 * r == pq) v.re -= 1.0;
-              x[r][C] = (NB * j + lc < N) ? v : cmk<double>(0, 0);
-            }
+          for (int c = 0; c < NC; ++c) {
+            const int i = lr + 16 * r, j = CPW * wv + lc + 4 * c;
+            const bool ok = i < N && j < N;
+            const zd v = to_zd(stage[ok ? i + j * N : 0]);
+            x[r][c] = ok ? v : cmk<double>(0, 0);
           }
-        });
+      } else {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const int i = lr + 16 * r, j = CPW * wv + lc + 4 * c;
+            const bool ok = i < N && j < N;
+            const zd v = to_zd(in[ok ? i + (long long)j * p.ld_in : 0]);
+            x[r][c] = ok ? v : cmk<double>(0, 0);
+          }
       }
-      if (j + 2 < NP) hand_over(j + 2, buf);
-      // this phase's reads of the row buffer before the next phase's writes
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else if (j + 1 < NP) {
-      factor(j + 1, true);
+      hand_over(0, 0);
+      if (NP > 1) hand_over(1, 1);
     }
-    __syncthreads();
-  }
 
-  if (!upd) __builtin_amdgcn_s_setprio(0);
-  for (int i = threadIdx.x; i < N; i += NT) qinv[perm[i]] = i;
-  // sign of the row permutation: parity of its inversions, counted in parallel
-  {
-    int odd = 0;
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-      const int i = e / N, j = e - i * N;
-      odd ^= (i < j && perm[i] > perm[j]) ? 1 : 0;
-    }
-    if (odd) atomicXor(&parity, 1);
-  }
-  // log|det| and the phase of det from the pivots: wave 0, fixed-order trees (the 1-D kernel's)
-  double lognorm = 0.0;
-  zd phase = cmk<double>(1.0, 0.0);
-  if (wv == 0) {
+    // the pivoting wave's chain is the critical path: its instructions go first on its SIMD
+    if (!upd) __builtin_amdgcn_s_setprio(2);
+    lds_barrier();
+    if (!upd) factor(0, false);
+    lds_barrier();
+    int sk = 0, sqj = q0j, sqt = q0t;  // the pivoting wave's next staging load
+    for (int j = 0; j < NP; ++j) {
+      const int buf = j & 1;
+      if (upd) {
+        int P[NB];
 #pragma unroll
-    for (int q = 0; q < (COLS + 63) / 64; ++q) {
-      const int k = lane + 64 * q;
-      if (k < N) {
-        const zd pv = pivs[k];
-        const double ap = cabs(pv);
-        const double ia = ap > 0.0 ? 1.0 / ap : 0.0;
-        lognorm += log(ap);
-        phase = cmul(phase, cmk<double>(pv.re * ia, pv.im * ia));
+        for (int k = 0; k < NB; ++k) P[k] = __builtin_amdgcn_readfirstlane(pp[buf][k]);
+        // the pivot rows of this wave's columns, written by the lanes holding them: one
+        // independent store per row register, the lane condition naming the register (branches
+        // that differ only in the register would be merged into a read through a run-time pointer)
+        // (also for the padding columns of the last panel, whose pivot row 0 keeps the buffer
+        // finite under their zero multipliers)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+          static_for<NR>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int R = decltype(rc)::value;
+            if (lr + 16 * R == P[k]) {
+#pragma unroll
+              for (int c = 0; c < NC; ++c) rb[wv][k][lc + 4 * c] = x[R][c];
+            }
+          });
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // x -= M_j rb, one k at a time (straight-line code, so that the loads of k+1 are issued
+        // under the multiply-adds of k; past N the multipliers are zero)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+          zd mk[NR], rk[NC];
+#pragma unroll
+          for (int r = 0; r < NR; ++r) mk[r] = mm[buf][k][lr + 16 * r];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) rk[c] = rb[wv][k][lc + 4 * c];
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) x[r][c] = cfms(x[r][c], mk[r], rk[c]);
+        }
+        // panel j's own columns become S = M - I[:, P] (zero past N)
+        if (j / NC == wv) {
+          const unsigned sel = col_mask(j);
+          static_for<NC>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int C = decltype(cc)::value;
+            if (sel & (1u << C)) {
+              const int pq = pp[buf][lc];
+#pragma unroll
+              for (int r = 0; r < NR; ++r) {
+                zd v = mm[buf][lc][lr + 16 * r];
+                if (lr + 16 * r == pq) v.re -= 1.0;
+                x[r][C] = (NB * j + lc < N) ? v : cmk<double>(0, 0);
+              }
+            }
+          });
+        }
+        if (j + 2 < NP) hand_over(j + 2, buf);
+        // this phase's reads of the row buffer before the next phase's writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      } else if (j + 1 < NP) {
+        if (stage_next) {
+          for (int u = 0; u < IPP && sk < NI; ++u, ++sk) {
+            stage_load(nb, sk, sqj, sqt);
+            // 64 chunks on: at most two column steps, a column holding at least 33 chunks
+            sqt += 64;
+            if (sqt >= CPC) { sqt -= CPC; ++sqj; }
+            if (sqt >= CPC) { sqt -= CPC; ++sqj; }
+          }
+        }
+        factor(j + 1, true);
+      }
+      lds_barrier();
+    }
+
+    if (!upd) __builtin_amdgcn_s_setprio(0);
+    for (int i = tid; i < N; i += NT) qinv[perm[i]] = i;
+    // sign of the row permutation: parity of its inversions, counted in parallel (pair e = i N + j,
+    // stepped by NT = di N + dj)
+    {
+      int odd = 0;
+      const int di = NT / N, dj = NT - di * N;
+      int i = tid / N, j = tid - (tid / N) * N;
+      for (int e = tid; e < N * N; e += NT) {
+        odd ^= (i < j && perm[i] > perm[j]) ? 1 : 0;
+        i += di;
+        j += dj;
+        if (j >= N) { j -= N; ++i; }
+      }
+      if (odd) __hip_atomic_fetch_xor(&parity, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    // det = +-(product of the pivots), kept as a complex mantissa z (larger part in [1/2, 1)) and a
+    // power of two: wave 0, lane products then a fixed-order tree; the logarithms are taken after
+    // the matrix loop
+    zd z = cmk<double>(1.0, 0.0);
+    int ez = 0;
+    auto renorm = [&]() __attribute__((always_inline)) {
+      int e;
+      (void)frexp(fmax(fabs(z.re), fabs(z.im)), &e);
+      z = cmk<double>(ldexp(z.re, -e), ldexp(z.im, -e));
+      ez += e;
+    };
+    if (wv == 0) {
+#pragma unroll
+      for (int q = 0; q < (COLS + 63) / 64; ++q) {
+        const int k = lane + 64 * q;
+        if (k < N) {
+          z = cmul(z, pivs[k]);
+          renorm();
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const zd zo = cmk<double>(__shfl_xor(z.re, o, 64), __shfl_xor(z.im, o, 64));
+        ez += __shfl_xor(ez, o, 64);
+        z = cmul(z, zo);
+        renorm();
       }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      lognorm += __shfl_xor(lognorm, o, 64);
-      const zd q = cmk<double>(__shfl_xor(phase.re, o, 64), __shfl_xor(phase.im, o, 64));
-      phase = cmul(phase, q);
+    lds_barrier();
+    if (tid == 0) {
+      sm.dz[m] = parity ? cmk<double>(-z.re, -z.im) : z;
+      sm.de[m] = singular ? GJB_SINGULAR : ez;
+      if (p.info) p.info[bidx] = singular;
     }
+    // inverse: O^{-1}[qinv[i]][perm[j]] = x[i][j] / (-pivot of row i)
+    if (p.write_inverse && upd && STAGE) {
+      // four columns at a time through the wave's part of the (now idle) panel buffers, rows put
+      // in output order there, so that 16 lanes store 16 consecutive rows of a column
+      ct* wb = reinterpret_cast<ct*>(&pc[0][0][0]) + wv * 4 * NMAX_B;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int i = lr + 16 * r;
+          if (i < N) wb[lc * NMAX_B + qinv[i]] = from_zd<T>(cmul(x[r][c], rowscale[i]));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int j = CPW * wv + lc + 4 * c;
+        if (j < N) {
+          ct* oc = out + (long long)perm[j] * p.ld_out;
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const int k = lr + 16 * r;
+            if (k < N) oc[k] = wb[lc * NMAX_B + k];
+          }
+        }
+        // the reads above before the next columns' writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    } else if (p.write_inverse && upd) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int i = lr + 16 * r;
+        if (i >= N) continue;
+        const int row = qinv[i];
+        const zd sc = rowscale[i];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const int j = CPW * wv + lc + 4 * c;
+          if (j >= N) continue;
+          out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(x[r][c], sc));
+        }
+      }
+    }
+    staged = stage_next;
+    ++m;
+    if (!STAGE) break;
   }
+  // no staging load may still be writing LDS when the workgroup ends
+  if (!upd) __builtin_amdgcn_s_waitcnt(0x0F70);
+  // the log-determinants of the workgroup's matrices, one per thread
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const zd ph = parity ? cmk<double>(-phase.re, -phase.im) : phase;
-    zd ld;
-    if (singular) {
-      ld = cmk<double>(-1e300, 0.0);
-    } else {
-      ld = cmk<double>(lognorm, atan2(ph.im, ph.re));
-      ld.re += log(hypot(ph.re, ph.im));
-    }
-    p.logdet[bidx] = ld;
-    if (p.info) p.info[bidx] = singular;
-  }
-  if (!p.write_inverse || !upd) return;
-  // inverse: O^{-1}[qinv[i]][perm[j]] = x[i][j] / (-pivot of row i)
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int i = lr + 16 * r;
-    if (i >= N) continue;
-    const int row = qinv[i];
-    const zd sc = rowscale[i];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int j = CPW * wv + lc + 4 * c;
-      if (j >= N) continue;
-      out[row + (long long)perm[j] * p.ld_out] = from_zd<T>(cmul(x[r][c], sc));
-    }
+  for (int q = threadIdx.x; q < m; q += NT) {
+    const int e = sm.de[q];
+    if (e == GJB_SKIPPED) continue;
+    const zd zq = sm.dz[q];
+    p.logdet[bk.x + q * bk.gx] = e == GJB_SINGULAR ? cmk<double>(-1e300, 0.0)
+                                                   : cmk<double>(fma((double)e, M_LN2, log(hypot(zq.re, zq.im))),
+                                                                 atan2(zq.im, zq.re));
   }
 }
 
@@ -1148,7 +1304,10 @@ int run_gj2(const SmallDenseArgs& a, hipStream_t s) {
 }
 template <typename T, int NR, int NC, int WV, int NRF, int WPE>
 int run_gjb(const SmallDenseArgs& a, hipStream_t s) {
-  return glaunch<GjBK<T, NR, NC, WV, NRF, WPE>>(a, dim3(a.batch), dim3(64 * (WV + 1)), 0, s);
+  typedef GjBK<T, NR, NC, WV, NRF, WPE> K;
+  // with staging, one workgroup per CU (its registers allow no more) walks over the matrices
+  const int grid = K::STAGE ? std::max(std::min(a.batch, cu_count()), (a.batch + GJB_MAXM - 1) / GJB_MAXM) : a.batch;
+  return glaunch<K>(a, dim3(grid), dim3(64 * (WV + 1)), 0, s);
 }
 template <typename T, int NR, int NC>
 int run_chol(const SmallDenseArgs& a, hipStream_t s) {

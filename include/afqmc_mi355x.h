@@ -389,6 +389,10 @@ int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const 
  * 0 the default for N and batch */
 int afqmc_test_gj_inverse_layout(afqmc_ctx ctx, int precision, int N, int batch, int layout, const double* A,
                                  double* Ainv, double* logdet, int* info);
+/* the same over the matrices b with alive[b] != 0 only; the others keep A (the inverse is written
+ * in place) and the logdet values passed in */
+int afqmc_test_gj_inverse_masked(afqmc_ctx ctx, int precision, int N, int batch, int layout, const int* alive,
+                                 const double* A, double* Ainv, double* logdet, int* info);
 /* device-to-device streaming copy of `bytes` (a multiple of 16), reps times: the reference kernel
  * for scaling HBM traffic counters; ms = average kernel time */
 int afqmc_test_copy(afqmc_ctx ctx, long long bytes, int reps, double* ms);

@@ -81,6 +81,20 @@ def gj_inverse_layout(A, layout, precision=0):
     return np.transpose(out, (0, 2, 1)), ld, info
 
 
+def gj_inverse_masked(A, alive, layout=0, precision=0, fill=0.0):
+    """gj_inverse over the matrices with alive != 0; the others keep A and logdet = fill."""
+    A = np.asarray(A, dtype=np.complex128)
+    batch, N, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    out = np.zeros_like(buf)
+    ld = np.full(batch, fill, dtype=np.complex128)
+    info = np.zeros(batch, dtype=np.int32)
+    al = np.ascontiguousarray(alive, dtype=np.int32)
+    check(lib().afqmc_test_gj_inverse_masked(ctx().h, precision, N, batch, layout, iptr(al), dptr(buf), dptr(out),
+                                             dptr(ld), iptr(info)))
+    return np.transpose(out, (0, 2, 1)), ld, info
+
+
 def chol_inverse(S):
     S = np.asarray(S, dtype=np.complex128)
     batch, N, _ = S.shape
