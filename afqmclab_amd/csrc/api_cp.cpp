@@ -403,10 +403,10 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
     cp->Mb[0] = 2 * L; cp->Nb[0] = d->N;
   }
   for (int b = 0; b < cp->nblk; ++b) {
-    if (cp->Nb[b] > 160) {
+    if (cp->Nb[b] > 256) {
       const int nb = cp->Nb[b];
       delete cp;
-      REQUIRE(false, "block column count %d exceeds the supported 160", nb);
+      REQUIRE(false, "block column count %d exceeds the supported 256", nb);
     }
   }
   cp->diag_only = (!hop && !mixing) ? 1 : 0;
@@ -475,6 +475,8 @@ int afqmc_cp_create(afqmc_ctx ctx, const afqmc_cp_desc* d, afqmc_cp* out) {
     const long long Mb = cp->Mb[b], Nb = cp->Nb[b];
     cp->ntile[b] = (int)((Nb + 63) / 64);
     rc = cp->ovl[b].alloc_bytes(es * Nb * Nb * nw);
+    if (!rc && small_dense_gj_work_bytes((int)Nb, nw) > 0)
+      rc = cp->gjwork[b].alloc_bytes(small_dense_gj_work_bytes((int)Nb, nw));
     if (!rc) rc = cp->ldet[b].alloc(nw);
     if (!rc) rc = cp->info[b].alloc(nw);
     for (int g = 0; g < 3 && !rc; ++g) rc = cp->gpart[g][b].alloc((size_t)nw * cp->ntile[b] * Mb);

@@ -83,7 +83,9 @@ int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ain
   DevBuf<cplx<T>> d;
   DevBuf<zd> ld;
   DevBuf<int> inf, al;
+  DevBuf<char> wk;
   int rc = d.alloc(n);
+  if (!rc && kind == 0 && small_dense_gj_work_bytes(N, batch) > 0) rc = wk.alloc_bytes(small_dense_gj_work_bytes(N, batch));
   if (!rc) rc = ld.alloc(batch);
   if (!rc) rc = inf.alloc(batch);
   if (!rc && alive) rc = al.alloc(batch);
@@ -97,6 +99,7 @@ int gj_t(afqmc_ctx ctx, int kind, int N, int batch, const double* A, double* Ain
   a.N = N; a.batch = batch; a.in = d.p; a.ld_in = N; a.stride_in = (long long)N * N;
   a.out = d.p; a.ld_out = N; a.stride_out = a.stride_in; a.logdet = ld.p; a.info = inf.p;
   a.write_inverse = 1; a.nsub = N; a.alive_div = 1; a.layout = layout; a.alive = alive ? al.p : nullptr;
+  a.work = wk.p;
   rc = kind == 0 ? launch_gj_inverse<T>(a, s) : launch_chol_inverse<T>(a, s);
   if (rc) return rc;
   std::vector<cplx<T>> o(n);

@@ -34,6 +34,7 @@ struct afqmc_cp_s {
   DevBuf<double> ubuf;
   // workspace
   DevBuf<char> ovl[2], scale[2], d2x2, qrS[2], qr64[2], qr64b[2];
+  DevBuf<char> gjwork[2];  // FP64 working copies of the overlap matrices above 160 columns
   DevBuf<zd> ldet[2], qrld[2], logO, gpart[3][2], per_walker, site_acc, site_part, den;
   DevBuf<int> info[2], qrinfo[2], prefix, table, comb_num, qr_need, qr_flag;
   DevBuf<double> weight, comb_prob, force_rec;
@@ -128,6 +129,7 @@ int gj_blocks(afqmc_cp cp, afqmc_walkers w, bool write_inverse, const int* alive
     s.out = cp->ovl[b].p; s.ld_out = s.ld_in; s.stride_out = s.stride_in;
     s.logdet = cp->ldet[b].p; s.info = cp->info[b].p; s.write_inverse = write_inverse ? 1 : 0;
     s.alive = alive; s.alive_div = 1;
+    s.work = cp->gjwork[b].p;
     int rc = launch_gj_inverse<T>(s, cp->ctx->stream);
     if (rc) return rc;
   }

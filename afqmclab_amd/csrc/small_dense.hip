@@ -34,6 +34,7 @@
 //   coalesced.  Returns sum log R_kk (CholeskyQR step of the walker restabilisation).  For 129
 //   to 160 columns the third column group (columns 128..159, lanes 0..31) lives in LDS, as in
 //   the Gauss-Jordan kernel.
+// From 161 to 256 columns both work on a copy of the matrix in device memory (GjWideK, CholWideK).
 #include "small_dense.hpp"
 
 #include "group.hpp"
@@ -1280,6 +1281,212 @@ __device__ __forceinline__ void CholK<T, NR, NC>::run(const BlockIdx3& bk, const
   }
 }
 
+// 161 to 256 columns.  An N x N complex FP64 matrix is then up to 1 MB, more than a CU's registers
+// and LDS hold together (512 KB + 160 KB), so one workgroup per matrix works on a copy in device
+// memory (the L2 holds the rows and columns of a step) with one barrier per phase of a step:
+// correct at these sizes, not tuned.  Thread t takes row t % 256 of columns t / 256 + 4c, so every
+// pass over the matrix is coalesced down the columns.
+constexpr int NWIDE = 256;
+constexpr int WIDE_CSTEP = 64 * WAVES / NWIDE;  // column phases of a 1024-thread workgroup
+
+// Gauss-Jordan exchange steps on the FP64 copy W (N x N per matrix in SmallDenseArgs::work), with
+// the pivots of the register kernels (largest |re|+|im| among rows not yet used, first row on
+// ties).  Exchanging column k with pivot row p: W[p][k] = 1/piv, W[p][j] = -W[p][j]/piv,
+// W[i][k] = W[i][k]/piv, W[i][j] -= W[i][k] W[p][j]/piv.  After N steps
+// O^{-1}[i][perm[j]] = W[perm[i]][j], perm[k] being column k's pivot row; the pivots equal LU's
+// u_kk, and the log-determinant is formed from them and the parity of perm as in the other kernels.
+template <typename T>
+struct GjWideK {
+  static constexpr int NT = 64 * WAVES, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const SmallDenseArgs& p);
+};
+template <typename T>
+__device__ __forceinline__ void GjWideK<T>::run(const BlockIdx3& bk, const SmallDenseArgs& p) {
+  const int bidx = bk.x;
+  if (p.alive && !p.alive[bidx / p.alive_div]) return;
+  const int N = p.N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  typedef cplx<T> ct;
+  const ct* in = reinterpret_cast<const ct*>(p.in) + (long long)bidx * p.stride_in;
+  ct* out = reinterpret_cast<ct*>(p.out) + (long long)bidx * p.stride_out;
+  zd* W = reinterpret_cast<zd*>(p.work) + (long long)bidx * N * N;
+  __shared__ zd prow[NWIDE], mcol[NWIDE], pivs[NWIDE];
+  __shared__ int perm[NWIDE], used[NWIDE];
+  __shared__ unsigned long long wkey[NWIDE / 64];
+  __shared__ int singular, parity, spr;
+  __shared__ zd sinv;
+  const int i = tid & (NWIDE - 1), c0 = tid / NWIDE;
+  if (i < N)
+    for (int j = c0; j < N; j += WIDE_CSTEP) W[i + (long long)j * N] = to_zd(in[i + (long long)j * p.ld_in]);
+  if (tid < NWIDE) used[tid] = 0;
+  if (tid == 0) { singular = 0; parity = 0; }
+  __syncthreads();
+  for (int k = 0; k < N; ++k) {
+    if (tid < NWIDE) {
+      unsigned long long key = 0;
+      if (tid < N && !used[tid]) key = pivot_key(W[tid + (long long)k * N], tid);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned hi = __shfl_xor((unsigned)(key >> 32), o, 64), lo = __shfl_xor((unsigned)key, o, 64);
+        const unsigned long long q = ((unsigned long long)hi << 32) | lo;
+        key = q > key ? q : key;
+      }
+      if (lane == 0) wkey[wv] = key;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long key = wkey[0];
+      for (int q = 1; q < (N + 63) / 64; ++q) key = wkey[q] > key ? wkey[q] : key;
+      const int pr = 255 - (int)(key & 0xFF);
+      const zd piv = W[pr + (long long)k * N];
+      const bool nz = abs1(piv) > 0.0;
+      const double id = nz ? recip(piv.re * piv.re + piv.im * piv.im) : 0.0;
+      sinv = cmk<double>(piv.re * id, -piv.im * id);
+      spr = pr; perm[k] = pr; used[pr] = 1; pivs[k] = piv;
+      if (!nz) singular = 1;
+    }
+    __syncthreads();
+    const int pr = spr;
+    const zd inv = sinv;
+    if (tid < N) {
+      prow[tid] = W[pr + (long long)tid * N];
+      mcol[tid] = tid == pr ? cmk<double>(0, 0) : cmul(W[tid + (long long)k * N], inv);
+    }
+    __syncthreads();
+    if (i < N) {
+      const zd mi = mcol[i];
+      for (int j = c0; j < N; j += WIDE_CSTEP) {
+        zd* e = W + i + (long long)j * N;
+        if (j == k) *e = i == pr ? inv : mi;
+        else if (i == pr) *e = cmul(cmk<double>(-prow[j].re, -prow[j].im), inv);
+        else *e = cfms(*e, mi, prow[j]);
+      }
+    }
+    __syncthreads();
+  }
+  // parity of the row permutation: inversions counted per row
+  if (tid < N) {
+    int odd = 0;
+    for (int j = tid + 1; j < N; ++j) odd ^= perm[tid] > perm[j] ? 1 : 0;
+    if (odd) atomicXor(&parity, 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double lognorm = 0.0;
+    zd phase = cmk<double>(1.0, 0.0);
+    for (int k = 0; k < N; ++k) {
+      const zd pv = pivs[k];
+      const double ap = cabs(pv);
+      const double ia = ap > 0.0 ? 1.0 / ap : 0.0;
+      lognorm += log(ap);
+      phase = cmul(phase, cmk<double>(pv.re * ia, pv.im * ia));
+    }
+    if (parity) phase = cmk<double>(-phase.re, -phase.im);
+    p.logdet[bidx] = singular ? cmk<double>(-1e300, 0.0)
+                              : cmk<double>(lognorm + log(hypot(phase.re, phase.im)), atan2(phase.im, phase.re));
+    if (p.info) p.info[bidx] = singular;
+  }
+  if (!p.write_inverse) return;
+  if (i < N) {
+    const int row = perm[i];
+    for (int j = c0; j < N; j += WIDE_CSTEP) out[i + (long long)perm[j] * p.ld_out] = from_zd<T>(W[row + (long long)j * N]);
+  }
+}
+
+// Cholesky S = R^H R and R^{-1} in place in the FP64 output (161 to 256 columns).  After the Jacobi
+// scaling S' = D^-1/2 S D^-1/2 (a non-positive diagonal counts as a breakdown and scales by 1), the
+// right-looking factorisation of S' = R'^H R' keeps R' in the upper triangle and the Hermitian
+// trailing matrix above its diagonal; a pivot of S' at or below pivot_tol is a breakdown and is
+// replaced by 1.  Thread i then forms column i of X = R'^{-1} by back substitution into row i of the
+// lower triangle, and the output is R^{-1} = D^-1/2 X, upper triangular, zero below.  logdet gets
+// (sum log R_kk, sum_{k<nsub} log R_kk), info the breakdown flag, as in CholK.
+struct CholWideK {
+  static constexpr int NT = 64 * WAVES, WPE = 1;
+  __device__ static void run(const BlockIdx3& bk, const SmallDenseArgs& p);
+};
+__device__ __forceinline__ void CholWideK::run(const BlockIdx3& bk, const SmallDenseArgs& p) {
+  const int bidx = bk.x;
+  if (p.alive && !p.alive[bidx / p.alive_div]) return;
+  const int N = p.N;
+  const long long ld = p.ld_out;
+  const int tid = threadIdx.x;
+  const zd* in = reinterpret_cast<const zd*>(p.in) + (long long)bidx * p.stride_in;
+  zd* W = reinterpret_cast<zd*>(p.out) + (long long)bidx * p.stride_out;
+  __shared__ double dsq[NWIDE], logr[NWIDE], skk_all[NWIDE], rdiag[NWIDE];
+  __shared__ zd rrow[NWIDE];
+  __shared__ int bad;
+  const int i = tid & (NWIDE - 1), c0 = tid / NWIDE;
+  if (tid == 0) bad = 0;
+  if (in != W && i < N)
+    for (int j = c0; j < N; j += WIDE_CSTEP) W[i + (long long)j * ld] = in[i + (long long)j * p.ld_in];
+  __syncthreads();
+  if (tid < N) {
+    double d = W[tid + (long long)tid * ld].re;
+    if (!(d > 0.0)) { bad = 1; d = 1.0; }
+    dsq[tid] = 1.0 / sqrt(d);
+    logr[tid] = 0.5 * log(d);
+  }
+  __syncthreads();
+  if (i < N)
+    for (int j = c0; j < N; j += WIDE_CSTEP) W[i + (long long)j * ld] = cscale(W[i + (long long)j * ld], dsq[i] * dsq[j]);
+  __syncthreads();
+  for (int k = 0; k < N; ++k) {
+    // row k of R': r'_kk = sqrt(s'_kk), r'_kj = s'_kj / r'_kk (every thread reads the same s'_kk)
+    double skk = W[k + (long long)k * ld].re;
+    const bool ok = skk > p.pivot_tol;
+    if (!ok) skk = 1.0;
+    const double rk = sqrt(skk);
+    if (tid == 0) {
+      skk_all[k] = skk;
+      rdiag[k] = rk;
+      if (!ok) bad = 1;
+    }
+    if (tid < N && tid > k) {
+      const zd v = cscale(W[k + (long long)tid * ld], 1.0 / rk);
+      W[k + (long long)tid * ld] = v;
+      rrow[tid] = v;
+    }
+    __syncthreads();
+    if (tid == k) W[k + (long long)k * ld] = cmk<double>(rk, 0.0);
+    // trailing upper triangle: s'_ij -= conj(r'_ki) r'_kj for k < i <= j
+    if (i < N && i > k) {
+      const zd ci = cconj(rrow[i]);
+      for (int j = c0; j < N; j += WIDE_CSTEP)
+        if (j >= i) W[i + (long long)j * ld] = cfms(W[i + (long long)j * ld], ci, rrow[j]);
+    }
+    __syncthreads();
+  }
+  // X = R'^{-1}: x_ii = 1 / r'_ii, x_ji = -(sum_{j<l<=i} r'_jl x_li) / r'_jj, into W[i + j ld]
+  if (tid < N) {
+    const int c = tid;
+    const double xd = 1.0 / rdiag[c];
+    for (int j = c - 1; j >= 0; --j) {
+      zd s = cmul(W[j + (long long)c * ld], cmk<double>(xd, 0.0));
+      for (int l = j + 1; l < c; ++l) s = cadd(s, cmul(W[j + (long long)l * ld], W[c + (long long)l * ld]));
+      W[c + (long long)j * ld] = cscale(s, -1.0 / rdiag[j]);
+    }
+  }
+  __syncthreads();
+  // R^{-1}(j, i) = dsq[j] x_ji above the diagonal, dsq[i] / r'_ii on it, zero below
+  if (i < N)
+    for (int j = c0; j < i; j += WIDE_CSTEP) {
+      const zd x = W[i + (long long)j * ld];
+      W[j + (long long)i * ld] = cscale(x, dsq[j]);
+      W[i + (long long)j * ld] = cmk<double>(0, 0);
+    }
+  if (tid < N) W[tid + (long long)tid * ld] = cmk<double>(dsq[tid] / rdiag[tid], 0.0);
+  if (tid == 0) {
+    double ls = 0.0, lsub = 0.0;
+    for (int k = 0; k < N; ++k) {
+      const double l = logr[k] + 0.5 * log(skk_all[k]);
+      ls += l;
+      if (k < p.nsub) lsub += l;
+    }
+    p.logdet[bidx] = cmk<double>(ls, lsub);
+    if (p.info) p.info[bidx] = bad;
+  }
+}
+
 // compute units of the current device (cached per device; threads racing on the first call all
 // store the same value)
 int cu_count() {
@@ -1316,9 +1523,20 @@ int run_chol(const SmallDenseArgs& a, hipStream_t s) {
 
 template <typename T>
 int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
-  REQUIRE(a.N >= 1 && a.N <= NMAX, "small dense kernels support 1 <= N <= %d (got %d)", NMAX, a.N);
+  REQUIRE(a.N >= 1 && a.N <= NWIDE, "small dense kernels support 1 <= N <= %d (got %d)", NWIDE, a.N);
   REQUIRE(a.alive_div >= 1, "alive_div must be >= 1");
   if (a.batch <= 0) return 0;
+  if (a.N > NMAX) {
+    // beyond the register-resident kernels: working copies in device memory
+    if (kind == 0) {
+      REQUIRE(a.work != nullptr, "Gauss-Jordan of %d columns needs a work buffer of %lld bytes", a.N,
+              small_dense_gj_work_bytes(a.N, a.batch));
+      return glaunch<GjWideK<T>>(a, dim3(a.batch), dim3(64 * WAVES), 0, s);
+    }
+    REQUIRE(sizeof(T) == 8, "Cholesky of %d columns runs in FP64 only", a.N);
+    REQUIRE(a.ld_out >= a.N && a.stride_out >= a.ld_out * a.N, "Cholesky of %d columns: output layout too small", a.N);
+    return glaunch<CholWideK>(a, dim3(a.batch), dim3(64 * WAVES), 0, s);
+  }
   if (kind == 0) {
     // Defaults, from 4096 (112, 96 columns), 2048 (64), 1024 (72) and 3072 (40, 24) inversions on
     // MI355X: the 2-D layout up to 32 columns (24: 0.070 ms against 0.101 for the 1-D layout; from
@@ -1375,6 +1593,10 @@ int dispatch(int kind, const SmallDenseArgs& a, hipStream_t s) {
 }
 
 }  // namespace
+
+long long small_dense_gj_work_bytes(int N, int batch) {
+  return N > NMAX ? (long long)sizeof(zd) * N * N * std::max(batch, 0) : 0;
+}
 
 template <typename T>
 int launch_gj_inverse(const SmallDenseArgs& a, hipStream_t s) { return dispatch<T>(0, a, s); }

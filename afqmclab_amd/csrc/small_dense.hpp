@@ -14,7 +14,11 @@ struct SmallDenseArgs {
   double pivot_tol;    // chol only: info = 1 also when a pivot of the Jacobi-scaled matrix is <= this
   int layout;          // gj only: 0 the default kernel for N, 1 the 1-D lane layout, 2 the 2-D lane layout,
                        // 3 panels of four columns (N <= 112)
+  void* work;          // gj with 161 <= N <= 256 only: FP64 complex working copies, N * N per matrix
+                       // of the batch (the matrix does not fit a CU's registers and LDS)
 };
+// bytes of SmallDenseArgs::work a Gauss-Jordan call of this size needs (0 up to 160 columns)
+long long small_dense_gj_work_bytes(int N, int batch);
 
 template <typename T> int launch_gj_inverse(const SmallDenseArgs& a, hipStream_t s);
 template <typename T> int launch_chol_inverse(const SmallDenseArgs& a, hipStream_t s);

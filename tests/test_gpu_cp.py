@@ -419,3 +419,60 @@ def test_hs_apply_2x2_shapes(lattice, precision):
         ref, rlw = niupnidn_apply_right(s, wfs[i], logw[i])
         assert np.max(np.abs(out[i] - ref)) < tol * max(1.0, np.max(np.abs(ref)))
         assert abs(lw[i] - rlw) < 1e-14
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dense_wide_walkers_step_stabilize_measure(precision):
+    """Dense walkers of 192 columns (12 x 12 lattice, 96 + 96 electrons), beyond the
+    register-resident Gauss-Jordan and Cholesky kernels: one step with injected uniforms, a QR and
+    a mixed measurement against the oracle.  FP64 to 1e-10; the mixed path (FP32 walkers, FP64
+    factorisations) to 1e-4 relative on log-weights and energies, fields exact."""
+    model, om, phi, prop, ws, wfs, logw, method = make_case("dense", nx=12, ny=12, nup=96, ndn=96, nw=4,
+                                                            precision=precision)
+    assert model.N == 192
+    tol = RTOL if precision == 0 else 1e-4
+    rng = np.random.default_rng(31)
+    # walkers near the trial, so that the constraint keeps all of them through the step
+    wfs = [phi + 0.03 * crand(rng, 2 * model.L, model.N) for _ in range(ws.nw)]
+    ws.upload(wfs, logw)
+    u = rng.uniform(0, 1, model.L * ws.nw)
+    prop.set_uniforms(u)
+    prop.set_ET(-150.0)
+    f_dev, aux_dev = prop.step(ws, record=True)
+    om_method = OracleMethod(dt=method.dt, decompType="densitySpin", walkerSizePerThread=ws.nw, ET=-150.0,
+                             thermalSize=0, ETAdjustMaxSize=0)
+    drv = OracleCP(om_method, om, ListStream(u), phiT=SD(wf=phi))
+    drv.walker = [SD(wf=wfs[i], logw=logw[i]) for i in range(ws.nw)]
+    drv.walkerIsAlive = [True] * ws.nw
+    drv.projectExpMinusDtKExpMinusDtV()
+    out, lw, al = ws.download()
+    recs = {r[0]: r for r in drv.records}
+    for i in range(ws.nw):
+        assert bool(al[i]) == drv.walkerIsAlive[i]
+        if not drv.walkerIsAlive[i]:
+            continue
+        assert np.array_equal(aux_dev[i], recs[i][2])
+        close_log(lw[i], drv.walker[i].logw, tol)
+    # QR of the stepped walkers (Cholesky of 192 columns)
+    prop.stabilize(ws)
+    out2, lw2, _ = ws.download()
+    for i in range(ws.nw):
+        if not drv.walkerIsAlive[i]:
+            continue
+        ref = SD(wf=out[i], logw=lw[i])
+        ref.stabilize()
+        close_log(lw2[i], ref.logw, tol)
+        q = out2[i]
+        assert np.max(np.abs(q.conj().T @ q - np.eye(model.N))) < (1e-12 if precision == 0 else 1e-5)
+    # mixed measurement on walkers with O(1) weights
+    ws.upload(wfs, np.zeros(ws.nw, dtype=complex))
+    prop.reset_measure()
+    prop.measure(ws)
+    e = prop.energy()
+    drv2 = OracleCP(om_method, om, UniformStream(1), phiT=SD(wf=phi))
+    drv2.walker = [SD(wf=wfs[i], logw=0j) for i in range(ws.nw)]
+    drv2.walkerIsAlive = [True] * ws.nw
+    drv2.addMixedMeasurement()
+    mm = drv2.mixedMeasure
+    for key, ref in [("den", mm.den), ("HNum", mm.HNum), ("KNum", mm.KNum), ("VNum", mm.VNum)]:
+        assert abs(e[key] - ref) < tol * max(1.0, abs(ref)), (key, e[key], ref)

@@ -119,7 +119,7 @@ def test_zgemm_batched_strided_and_scaled(precision):
             assert np.max(np.abs(C[b][:, cols] - ref)) < TOL[precision] * 30
 
 
-@pytest.mark.parametrize("N", [1, 3, 10, 16, 17, 40, 64, 65, 112, 128, 129, 150, 160])
+@pytest.mark.parametrize("N", [1, 3, 10, 16, 17, 40, 64, 65, 112, 128, 129, 150, 160, 161, 176, 224, 256])
 def test_gj_inverse_and_logdet(N):
     rng = np.random.default_rng(N)
     batch = 6
@@ -134,7 +134,7 @@ def test_gj_inverse_and_logdet(N):
         assert -np.pi < ld[b].imag <= np.pi
 
 
-@pytest.mark.parametrize("N", [112, 150])
+@pytest.mark.parametrize("N", [112, 150, 224])
 def test_gj_inverse_fp32_storage(N):
     """FP32 input and output, FP64 elimination (the mixed path): the inverse to FP32 rounding and the
     log-determinant of the FP32-rounded matrix to FP64 accuracy."""
@@ -288,7 +288,28 @@ def test_gj_pivoting_cases():
     assert np.allclose(inv[2], np.eye(8)) and abs(ld[2]) == 0
 
 
-@pytest.mark.parametrize("N", [5, 28, 64, 112, 129, 160])
+def test_gj_wide_pivoting_singular_and_mask():
+    """161 to 256 columns (working copies in device memory): a row permutation with an odd number
+    of swaps, an exactly singular matrix, a skipped matrix and a general one against LAPACK."""
+    N = 200
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(N)
+    P = np.eye(N, dtype=complex)[perm]
+    S = np.ones((N, N), dtype=complex)
+    A = crand(rng, 1, N, N)[0] + 0.5 * np.eye(N)
+    inv, ld, info = gj_inverse_masked(np.stack([P, S, A, A]), np.array([1, 1, 0, 1], dtype=np.int32), fill=7.0)
+    sign = np.linalg.det(P).real
+    assert np.allclose(inv[0], P.T) and info[0] == 0
+    assert abs(np.exp(ld[0]) - sign) < 1e-12
+    assert info[1] == 1 and ld[1].real == -1e300
+    assert ld[2] == 7.0 and np.array_equal(inv[2], A)
+    ref = linalg.log_determinant(*linalg.lu_construct(A))
+    assert info[3] == 0 and np.max(np.abs(inv[3] @ A - np.eye(N))) < 1e-11 * N
+    assert abs(ld[3].real - ref.real) < 1e-11 * abs(ref.real)
+    assert abs(np.exp(1j * (ld[3].imag - ref.imag)) - 1) < 1e-11
+
+
+@pytest.mark.parametrize("N", [5, 28, 64, 112, 129, 160, 176, 224, 256])
 def test_chol_inverse(N):
     rng = np.random.default_rng(100 + N)
     batch = 4
