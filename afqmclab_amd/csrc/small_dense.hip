@@ -1288,6 +1288,7 @@ __device__ __forceinline__ void CholK<T, NR, NC>::run(const BlockIdx3& bk, const
 // pass over the matrix is coalesced down the columns.
 constexpr int NWIDE = 256;
 constexpr int WIDE_CSTEP = 64 * WAVES / NWIDE;  // column phases of a 1024-thread workgroup
+constexpr int WIDE_UNROLL = 8;                   // columns a thread loads before it stores
 
 // Gauss-Jordan exchange steps on the FP64 copy W (N x N per matrix in SmallDenseArgs::work), with
 // the pivots of the register kernels (largest |re|+|im| among rows not yet used, first row on
@@ -1354,12 +1355,25 @@ __device__ __forceinline__ void GjWideK<T>::run(const BlockIdx3& bk, const Small
     }
     __syncthreads();
     if (i < N) {
+      // eight independent loads in flight per thread before the stores
       const zd mi = mcol[i];
-      for (int j = c0; j < N; j += WIDE_CSTEP) {
-        zd* e = W + i + (long long)j * N;
-        if (j == k) *e = i == pr ? inv : mi;
-        else if (i == pr) *e = cmul(cmk<double>(-prow[j].re, -prow[j].im), inv);
-        else *e = cfms(*e, mi, prow[j]);
+      for (int j0 = c0; j0 < N; j0 += WIDE_UNROLL * WIDE_CSTEP) {
+        zd v[WIDE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < WIDE_UNROLL; ++u) {
+          const int j = j0 + u * WIDE_CSTEP;
+          if (j < N) v[u] = W[i + (long long)j * N];
+        }
+#pragma unroll
+        for (int u = 0; u < WIDE_UNROLL; ++u) {
+          const int j = j0 + u * WIDE_CSTEP;
+          if (j >= N) continue;
+          zd r;
+          if (j == k) r = i == pr ? inv : mi;
+          else if (i == pr) r = cmul(cmk<double>(-prow[j].re, -prow[j].im), inv);
+          else r = cfms(v[u], mi, prow[j]);
+          W[i + (long long)j * N] = r;
+        }
       }
     }
     __syncthreads();
@@ -1451,8 +1465,19 @@ __device__ __forceinline__ void CholWideK::run(const BlockIdx3& bk, const SmallD
     // trailing upper triangle: s'_ij -= conj(r'_ki) r'_kj for k < i <= j
     if (i < N && i > k) {
       const zd ci = cconj(rrow[i]);
-      for (int j = c0; j < N; j += WIDE_CSTEP)
-        if (j >= i) W[i + (long long)j * ld] = cfms(W[i + (long long)j * ld], ci, rrow[j]);
+      for (int j0 = c0; j0 < N; j0 += WIDE_UNROLL * WIDE_CSTEP) {
+        zd v[WIDE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < WIDE_UNROLL; ++u) {
+          const int j = j0 + u * WIDE_CSTEP;
+          if (j >= i && j < N) v[u] = W[i + (long long)j * ld];
+        }
+#pragma unroll
+        for (int u = 0; u < WIDE_UNROLL; ++u) {
+          const int j = j0 + u * WIDE_CSTEP;
+          if (j >= i && j < N) W[i + (long long)j * ld] = cfms(v[u], ci, rrow[j]);
+        }
+      }
     }
     __syncthreads();
   }
@@ -1462,6 +1487,7 @@ __device__ __forceinline__ void CholWideK::run(const BlockIdx3& bk, const SmallD
     const double xd = 1.0 / rdiag[c];
     for (int j = c - 1; j >= 0; --j) {
       zd s = cmul(W[j + (long long)c * ld], cmk<double>(xd, 0.0));
+#pragma unroll 8
       for (int l = j + 1; l < c; ++l) s = cadd(s, cmul(W[j + (long long)l * ld], W[c + (long long)l * ld]));
       W[c + (long long)j * ld] = cscale(s, -1.0 / rdiag[j]);
     }
