@@ -1288,7 +1288,7 @@ __device__ __forceinline__ void CholK<T, NR, NC>::run(const BlockIdx3& bk, const
 // pass over the matrix is coalesced down the columns.
 constexpr int NWIDE = 256;
 constexpr int WIDE_CSTEP = 64 * WAVES / NWIDE;  // column phases of a 1024-thread workgroup
-constexpr int WIDE_UNROLL = 8;                   // columns a thread loads before it stores
+constexpr int WIDE_UNROLL = 8;  // columns a thread loads before it stores (16 spills at the 64-VGPR limit, 2x slower)
 
 // Gauss-Jordan exchange steps on the FP64 copy W (N x N per matrix in SmallDenseArgs::work), with
 // the pivots of the register kernels (largest |re|+|im| among rows not yet used, first row on

@@ -69,7 +69,9 @@ typedef struct {
   int Nup;               /* spin_blocks: first Nup columns are spin up */
   int spin_blocks;       /* 1: walkers and trial are block diagonal (rows 0..L-1 x cols 0..Nup-1 and
                             rows L..2L-1 x cols Nup..N-1), K and the HS field do not mix spins;
-                            stored and multiplied as two blocks.  0: dense 2L x N. */
+                            stored and multiplied as two blocks.  0: dense 2L x N.  Each block
+                            has at most 256 columns (161 to 256: the factorisations work on
+                            copies in device memory, N^2 x 16 bytes per walker). */
   int precision;         /* 0: complex FP64 throughout; 1: complex FP32 walkers and GEMMs, FP64 LU,
                             log-determinants, QR, weights and energies; 2: as 1, with the
                             propagation GEMM in split-BF16 products (experimental) */
@@ -380,7 +382,7 @@ int afqmc_ph_adjust_ET(afqmc_ph ph);
 int afqmc_test_zgemm(afqmc_ctx ctx, int precision, int opa, int M, int N, int K, int batch, const double* A,
                      long long strideA, const double* B, const double* scale, int scale_group, int flags, int tile,
                      double* C);
-/* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting) */
+/* batched N x N inverse and log-determinant (Gauss-Jordan, partial pivoting), 1 <= N <= 256 */
 int afqmc_test_gj_inverse(afqmc_ctx ctx, int precision, int N, int batch, const double* A, double* Ainv,
                           double* logdet, int* info);
 /* the same with the kernel chosen: 1 lanes = rows (readlanes for the pivot row), 2 a 2-D lane layout
